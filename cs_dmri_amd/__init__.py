@@ -1,0 +1,10 @@
+"""cs_dmri_amd — MI355X (gfx950) 3D-SHORE fit with the interface of PennLINC/cs_dmri.
+
+The compute path is ``libshorefit.so`` (hand-written HIP kernels behind the C
+ABI in include/shorefit.h); there is no CPU fallback.
+"""
+
+from .models import design_matrix, fista_batch, n_coeffs, shore_fit, tikhonov_batch  # noqa: F401
+from .shore import GradientTable, ShoreFit, ShoreModel  # noqa: F401
+
+__version__ = "0.1.0"

@@ -1,0 +1,669 @@
+// C ABI of the shorefit library: validation and dispatch of the SHORE fit
+// (src/fit.rs:545-805), gradient-table handling (src/qspace.rs:90-227), and
+// the device set-up / fit launches.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace shorefit {
+
+namespace {
+thread_local std::string g_last_error;
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kTortoiseGmax = 80e-3;      // src/qspace.rs:15
+constexpr double kGyro = 267.51532e6;        // src/qspace.rs:18
+constexpr double kShellTolerance = 50.0;     // src/fit.rs:618
+constexpr size_t kMinShoreShells = 2;        // src/fit.rs:614
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+// Padded coefficient count: the smallest instantiated multiple of 16.
+int pad_coeffs(int K) {
+    static const int mbs[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+    for (int mb : mbs)
+        if (16 * mb >= K) return 16 * mb;
+    return -1;
+}
+int pad16(int n) { return (n + 15) / 16 * 16; }
+
+// src/qspace.rs:213-227 — TORTOISE's (Delta, delta) heuristic.
+void tortoise_deltas(const double* bvals, int64_t n, double* big, double* small) {
+    double mb = 0.0;
+    for (int64_t i = 0; i < n; ++i) mb = std::max(mb, bvals[i]);
+    if (mb <= 0.0) {
+        *big = 3e-3;
+        *small = 1e-3;
+        return;
+    }
+    const double temp = mb / kGyro / kGyro / kTortoiseGmax / kTortoiseGmax / 2.0 * 1e6;
+    const double small_ms = std::pow(temp, 1.0 / 3.0) * 1000.0;
+    *big = small_ms * 3.0 * 1e-3;
+    *small = small_ms * 1e-3;
+}
+
+// src/qspace.rs:171-197 + src/fit.rs:615-617 — greedy shell clustering.
+std::vector<double> dwi_shells(const double* bvals, int64_t n, double b0_thr) {
+    std::vector<double> b;
+    std::vector<double> cnt;
+    for (int64_t i = 0; i < n; ++i) {
+        if (bvals[i] <= b0_thr) continue;
+        bool placed = false;
+        for (size_t s = 0; s < b.size(); ++s)
+            if (std::fabs(b[s] - bvals[i]) <= kShellTolerance) {
+                b[s] = (b[s] * cnt[s] + bvals[i]) / (cnt[s] + 1.0);
+                cnt[s] += 1.0;
+                placed = true;
+                break;
+            }
+        if (!placed) {
+            b.push_back(bvals[i]);
+            cnt.push_back(1.0);
+        }
+    }
+    std::stable_sort(b.begin(), b.end());
+    return b;
+}
+
+std::string fmt_b(double b) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.0f", b);
+    return buf;
+}
+
+struct Warnings {
+    std::vector<std::string> msgs;
+    void to(ShorefitOutput* out) const {
+        out->n_warnings = 0;
+        size_t pos = 0;
+        std::memset(out->warnings, 0, sizeof(out->warnings));
+        for (const auto& m : msgs) {
+            if (pos + m.size() + 1 > sizeof(out->warnings)) break;
+            std::memcpy(out->warnings + pos, m.c_str(), m.size() + 1);
+            pos += m.size() + 1;
+            out->n_warnings++;
+        }
+    }
+};
+
+// Device-side design and its derived operands.
+struct Design {
+    int N = 0, K = 0, Np = 0, Kp = 0;
+    DevBuf<double> A;   // N x K row-major, FP64
+    DevBuf<double> G;   // K x K
+    double L = 0.0;     // Lipschitz constant sigma_max^2 / N (floored at eps)
+};
+
+void upload_design(Design& d, const double* A_host, int N, int K, hipStream_t s) {
+    d.N = N;
+    d.K = K;
+    d.Np = pad16(N);
+    d.Kp = pad_coeffs(K);
+    if (d.Kp < 0)
+        throw Error{SHOREFIT_E_UNSUPPORTED, "at most 256 coefficients (radial order 10) are supported, got " +
+                                                std::to_string(K)};
+    d.A.alloc(size_t(N) * K);
+    SF_HIP(hipMemcpyAsync(d.A.p, A_host, sizeof(double) * N * K, hipMemcpyHostToDevice, s));
+}
+
+void build_design(Design& d, const ShorefitGradients& g, double big, double small, uint32_t order, double zeta,
+                  hipStream_t s) {
+    const int N = int(g.n_grads);
+    const int K = n_coeffs(order);
+    d.N = N;
+    d.K = K;
+    d.Np = pad16(N);
+    d.Kp = pad_coeffs(K);
+    if (d.Kp < 0)
+        throw Error{SHOREFIT_E_UNSUPPORTED,
+                    "radial order " + std::to_string(order) + " (" + std::to_string(K) +
+                        " coefficients) exceeds the supported maximum of 10 (256 coefficients)"};
+    // src/qspace.rs:130-149 — q = sqrt(b / (4 pi^2 tau)), 0 at or below the b0 threshold.
+    std::vector<double> q(N);
+    const double tau = big - small / 3.0;
+    const double denom = 4.0 * kPi * kPi * tau;
+    for (int i = 0; i < N; ++i) q[i] = (g.bvals[i] <= g.b0_threshold) ? 0.0 : std::sqrt(g.bvals[i] / denom);
+    DevBuf<double> dq(N), dv(size_t(3) * N);
+    SF_HIP(hipMemcpyAsync(dq.p, q.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(dv.p, g.bvecs, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+    d.A.alloc(size_t(N) * K);
+    launch_design_matrix(dq.p, dv.p, N, order, zeta, d.A.p, s);
+    SF_HIP(hipStreamSynchronize(s));  // dq / dv are released at scope exit
+}
+
+void gram_and_lipschitz(Design& d, hipStream_t s) {
+    d.G.alloc(size_t(d.K) * d.K);
+    launch_gram(d.A.p, d.N, d.K, d.G.p, s);
+    DevBuf<double> lam(1);
+    launch_power_iteration(d.G.p, d.K, lam.p, s);
+    double lh = 0.0;
+    SF_HIP(hipMemcpyAsync(&lh, lam.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    // src/solver/fista.rs:51,58 — L = sigma_max^2 / n, floored at f64::EPSILON.
+    d.L = std::max(std::max(lh, 0.0) / double(d.N), std::numeric_limits<double>::epsilon());
+}
+
+struct FistaOps {
+    DevBuf<float> M, AT, A;
+    DevBuf<uint16_t> Mlow;
+    double step = 0.0;
+};
+
+void pack_fista(const Design& d, FistaOps& ops, int precision, hipStream_t s) {
+    ops.step = 1.0 / d.L;
+    const double c = ops.step / double(d.N);
+    ops.M.alloc(size_t(d.Kp) * d.Kp);
+    ops.AT.alloc(size_t(d.Kp) * d.Np);
+    ops.A.alloc(size_t(d.Np) * d.Kp);
+    launch_pack_M(d.G.p, d.K, d.Kp, c, ops.M.p, s);
+    launch_pack_rowblock(d.A.p, d.K, d.N, d.K, /*transpose=*/true, c, d.Kp, d.Np, ops.AT.p, s);
+    launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
+    (void)precision;
+}
+
+// P = (A^T A + lambda_n D_n + lambda_l D_l)^-1 A^T in FP64 (Cholesky on the
+// device), packed as the FP32 prologue operand.
+struct TikOps {
+    DevBuf<float> P, A;
+};
+
+void pack_tikhonov(const Design& d, const double* dn, const double* dl, double lam_n, double lam_l, TikOps& ops,
+                   hipStream_t s);
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        set_error("");
+        f();
+        return SHOREFIT_OK;
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SHOREFIT_E_DEVICE;
+    }
+}
+
+}  // namespace
+
+// FP64 host-side triangular solves are avoided: the columns of A^T are solved
+// against the Cholesky factor on the device.
+void launch_chol_solve_columns(const double* d_L, int K, const double* d_A, int N, double* d_P, hipStream_t s);
+
+namespace {
+void pack_tikhonov(const Design& d, const double* dn, const double* dl, double lam_n, double lam_l, TikOps& ops,
+                   hipStream_t s) {
+    // src/solver/tikhonov.rs:33-44
+    std::vector<double> G(size_t(d.K) * d.K);
+    SF_HIP(hipMemcpyAsync(G.data(), d.G.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < d.K; ++i) {
+        if (dn) G[size_t(i) * d.K + i] += lam_n * dn[i];
+        if (dl) G[size_t(i) * d.K + i] += lam_l * dl[i];
+    }
+    DevBuf<double> Lf(G.size());
+    DevBuf<int> status(1);
+    SF_HIP(hipMemcpyAsync(Lf.p, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice, s));
+    launch_cholesky(Lf.p, d.K, status.p, s);
+    int st = 0;
+    SF_HIP(hipMemcpyAsync(&st, status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    if (st != 0)
+        throw Error{SHOREFIT_E_NOT_PD,
+                    "normal-equations matrix should be PD (Cholesky pivot " + std::to_string(st - 1) +
+                        " is not positive)"};
+    DevBuf<double> P(size_t(d.K) * d.N);  // K x N row-major
+    launch_chol_solve_columns(Lf.p, d.K, d.A.p, d.N, P.p, s);
+    ops.P.alloc(size_t(d.Kp) * d.Np);
+    ops.A.alloc(size_t(d.Np) * d.Kp);
+    launch_pack_rowblock(P.p, d.K, d.N, d.N, /*transpose=*/false, 1.0, d.Kp, d.Np, ops.P.p, s);
+    launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
+    SF_HIP(hipStreamSynchronize(s));
+}
+
+void regularization_diagonals(uint32_t order, std::vector<double>& dn, std::vector<double>& dl) {
+    // src/basis/shore.rs:180-195
+    dn.clear();
+    dl.clear();
+    for (uint32_t n = 0; n <= order; ++n)
+        for (uint32_t l = 0; l <= n; l += 2)
+            for (uint32_t m = 0; m < 2 * l + 1; ++m) {
+                const double nn = double(n * (n + 1)), ll = double(l * (l + 1));
+                dn.push_back(nn * nn);
+                dl.push_back(ll * ll);
+            }
+}
+
+}  // namespace
+}  // namespace shorefit
+
+using namespace shorefit;
+
+extern "C" {
+
+const char* shorefit_last_error(void) { return g_last_error.c_str(); }
+
+const char* shorefit_version(void) { return "shorefit 0.1.0 (gfx950)"; }
+
+int shorefit_n_coeffs(uint32_t radial_order) {
+    if (radial_order % 2 != 0) return -1;
+    return n_coeffs(radial_order);
+}
+
+void shorefit_spec_default(ShorefitSpec* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->radial_order = 6;
+    s->zeta = 700.0;
+    s->regularization = SHOREFIT_REG_L1;
+    s->alpha_mode = SHOREFIT_ALPHA_L2_ANCHORED;
+    s->alpha = 1.0;
+    s->alpha_ratio = 1e-3;
+    s->path_n_alphas = 20;
+    s->path_eps = 0.0;
+    s->slack = 0.05;
+    s->max_iter = 1000;
+    s->tol = 1e-6;
+    s->non_negative = 0;
+    s->lambda_n = 1e-8;
+    s->lambda_l = 1e-8;
+    s->compute_diagnostics = 0;
+    s->allow_single_shell = 0;
+    s->precision = SHOREFIT_PREC_FP32;
+    s->refresh_every = 10;
+}
+
+int shorefit_design_matrix(const ShorefitGradients* g, uint32_t radial_order, double zeta, double* A_out) {
+    return guarded([&] {
+        if (!g || !A_out) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        double big = g->big_delta, small = g->small_delta;
+        if (std::isnan(big) || std::isnan(small)) tortoise_deltas(g->bvals, g->n_grads, &big, &small);
+        Design d;
+        build_design(d, *g, big, small, radial_order, zeta, nullptr);
+        SF_HIP(hipMemcpy(A_out, d.A.p, sizeof(double) * d.N * d.K, hipMemcpyDeviceToHost));
+    });
+}
+
+int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const float* signals, int64_t n_signals,
+                         double alpha, uint32_t max_iter, double tol, int32_t non_negative, int32_t precision,
+                         int32_t refresh_every, float* coef, uint32_t* iterations, uint8_t* converged,
+                         float* residual_l2, double* lipschitz_out) {
+    return guarded([&] {
+        if (!A || !signals || !coef || n_rows <= 0 || n_cols <= 0 || n_signals < 0)
+            throw Error{SHOREFIT_E_INVALID, "invalid fista_batch arguments"};
+        hipStream_t s = nullptr;
+        Design d;
+        upload_design(d, A, n_rows, n_cols, s);
+        gram_and_lipschitz(d, s);
+        if (lipschitz_out) *lipschitz_out = d.L;
+        FistaOps ops;
+        pack_fista(d, ops, precision, s);
+        const int64_t V = n_signals;
+        DevBuf<float> S(size_t(V) * n_rows), C(size_t(V) * n_cols), R(V);
+        DevBuf<uint32_t> It(V);
+        DevBuf<uint8_t> Cv(V);
+        if (V) SF_HIP(hipMemcpyAsync(S.p, signals, sizeof(float) * V * n_rows, hipMemcpyHostToDevice, s));
+        FistaLaunch L{};
+        L.data = S.p;
+        L.vox = nullptr;
+        L.n_vox = V;
+        L.N = d.N;
+        L.sig_stride = d.N;
+        L.K = d.K;
+        L.Kp = d.Kp;
+        L.Np = d.Np;
+        L.Mfrag = ops.M.p;
+        L.ATfrag = ops.AT.p;
+        L.Afrag = ops.A.p;
+        L.step = float(ops.step);
+        L.thr = float(ops.step * alpha);
+        L.ratio = 0.f;
+        L.alpha = alpha;
+        L.max_iter = max_iter;
+        L.tol = float(tol);
+        L.nonneg = non_negative;
+        L.precision = precision;
+        L.refresh = refresh_every;
+        L.coef = C.p;
+        L.resid = R.p;
+        L.iters = It.p;
+        L.converged = Cv.p;
+        launch_fista(L, s);
+        SF_HIP(hipStreamSynchronize(s));
+        if (V) {
+            SF_HIP(hipMemcpy(coef, C.p, sizeof(float) * V * n_cols, hipMemcpyDeviceToHost));
+            if (iterations) SF_HIP(hipMemcpy(iterations, It.p, sizeof(uint32_t) * V, hipMemcpyDeviceToHost));
+            if (converged) SF_HIP(hipMemcpy(converged, Cv.p, V, hipMemcpyDeviceToHost));
+            if (residual_l2) SF_HIP(hipMemcpy(residual_l2, R.p, sizeof(float) * V, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int shorefit_tikhonov_batch(const double* A, int32_t n_rows, int32_t n_cols, const double* d_n, const double* d_l,
+                            double lambda_n, double lambda_l, const float* signals, int64_t n_signals, float* coef,
+                            float* residual_l2) {
+    return guarded([&] {
+        if (!A || !signals || !coef || n_rows <= 0 || n_cols <= 0 || n_signals < 0)
+            throw Error{SHOREFIT_E_INVALID, "invalid tikhonov_batch arguments"};
+        hipStream_t s = nullptr;
+        Design d;
+        upload_design(d, A, n_rows, n_cols, s);
+        d.G.alloc(size_t(d.K) * d.K);
+        launch_gram(d.A.p, d.N, d.K, d.G.p, s);
+        TikOps ops;
+        pack_tikhonov(d, d_n, d_l, lambda_n, lambda_l, ops, s);
+        const int64_t V = n_signals;
+        DevBuf<float> S(size_t(V) * n_rows), C(size_t(V) * n_cols), R(V);
+        if (V) SF_HIP(hipMemcpyAsync(S.p, signals, sizeof(float) * V * n_rows, hipMemcpyHostToDevice, s));
+        TikhonovLaunch L{};
+        L.data = S.p;
+        L.n_vox = V;
+        L.N = d.N;
+        L.sig_stride = d.N;
+        L.K = d.K;
+        L.Kp = d.Kp;
+        L.Np = d.Np;
+        L.Pfrag = ops.P.p;
+        L.Afrag = ops.A.p;
+        L.coef = C.p;
+        L.resid = R.p;
+        launch_tikhonov(L, s);
+        SF_HIP(hipStreamSynchronize(s));
+        if (V) {
+            SF_HIP(hipMemcpy(coef, C.p, sizeof(float) * V * n_cols, hipMemcpyDeviceToHost));
+            if (residual_l2) SF_HIP(hipMemcpy(residual_l2, R.p, sizeof(float) * V, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitSpec* spec,
+                 ShorefitOutput* out, void* stream) {
+    return guarded([&] {
+        if (!grads || !vol || !spec || !out) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        if (!out->coefficients) throw Error{SHOREFIT_E_INVALID, "coefficients output is required"};
+        hipStream_t s = as_stream(stream);
+        const ShorefitSpec& sp = *spec;
+        Warnings warn;
+
+        // --- spec validation (build_alpha_strategy, src/fit.rs:555-591; the
+        // binding's own checks, python/src/models.rs:117-147) -------------
+        if (sp.regularization == SHOREFIT_REG_NONNEG)
+            throw Error{SHOREFIT_E_UNSUPPORTED, "regularization 'nonneg' (ICLS) is not implemented on this device"};
+        if (sp.regularization != SHOREFIT_REG_L1 && sp.regularization != SHOREFIT_REG_L2)
+            throw Error{SHOREFIT_E_INVALID, "regularization must be l1, l2 or nonneg"};
+        double path_eps = sp.path_eps;
+        if (sp.regularization == SHOREFIT_REG_L1) {
+            switch (sp.alpha_mode) {
+                case SHOREFIT_ALPHA_FIXED: break;
+                case SHOREFIT_ALPHA_RATIO:
+                    if (!(sp.alpha_ratio > 0.0 && sp.alpha_ratio < 1.0))
+                        throw Error{SHOREFIT_E_INVALID,
+                                    "--alpha-ratio must lie in (0, 1), got " + std::to_string(sp.alpha_ratio)};
+                    break;
+                case SHOREFIT_ALPHA_PATH_BIC:
+                case SHOREFIT_ALPHA_L2_ANCHORED: {
+                    if (sp.path_n_alphas < 2)
+                        throw Error{SHOREFIT_E_INVALID,
+                                    "--path-n-alphas must be ≥ 2, got " + std::to_string(sp.path_n_alphas)};
+                    if (!(path_eps > 0.0)) path_eps = sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC ? 1e-3 : 1e-4;
+                    if (!(path_eps > 0.0 && path_eps < 1.0))
+                        throw Error{SHOREFIT_E_INVALID, "--path-eps must lie in (0, 1), got " + std::to_string(path_eps)};
+                    if (sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED && sp.slack < 0.0)
+                        throw Error{SHOREFIT_E_INVALID, "--slack must be ≥ 0, got " + std::to_string(sp.slack)};
+                    break;
+                }
+                default:
+                    throw Error{SHOREFIT_E_INVALID, "alpha_mode must be fixed, alpha-ratio, path-bic or l2-anchored"};
+            }
+        }
+        if (sp.radial_order % 2 != 0 || sp.radial_order == 0)
+            throw Error{SHOREFIT_E_INVALID,
+                        "radial_order must be a positive even number, got " + std::to_string(sp.radial_order)};
+        if (sp.precision != SHOREFIT_PREC_FP32 && sp.precision != SHOREFIT_PREC_FP16 &&
+            sp.precision != SHOREFIT_PREC_BF16)
+            throw Error{SHOREFIT_E_INVALID, "precision must be fp32, fp16 or bf16"};
+
+        // --- gradient table (src/qspace.rs:90-120) ------------------------
+        const int64_t N = grads->n_grads;
+        if (N <= 0 || !grads->bvals || !grads->bvecs) throw Error{SHOREFIT_E_INVALID, "empty gradient table"};
+        if (vol->nx < 0 || vol->ny < 0 || vol->nz < 0) throw Error{SHOREFIT_E_INVALID, "negative volume shape"};
+        double big = grads->big_delta, small = grads->small_delta;
+        out->delta_source = 0;
+        if (std::isnan(big) || std::isnan(small)) {
+            tortoise_deltas(grads->bvals, N, &big, &small);
+            out->delta_source = 1;
+        }
+        out->big_delta = big;
+        out->small_delta = small;
+        out->tau = big - small / 3.0;
+
+        // --- fit_shore refusals and warnings (src/fit.rs:691-730) ---------
+        const auto shells = dwi_shells(grads->bvals, N, grads->b0_threshold);
+        out->n_shells = int32_t(shells.size());
+        for (size_t i = 0; i < shells.size() && i < 32; ++i) out->dwi_shells[i] = shells[i];
+        if (shells.empty())
+            throw Error{SHOREFIT_E_FIT,
+                        "3D-SHORE needs diffusion-weighted data; every volume is at or below the b=0 threshold"};
+        if (shells.size() < kMinShoreShells) {
+            const std::string what = shells.size() == 1 ? "a single shell (b = " + fmt_b(shells[0]) + " s/mm²)"
+                                                        : std::to_string(shells.size()) + " shells";
+            if (!sp.allow_single_shell)
+                throw Error{
+                    SHOREFIT_E_FIT,
+                    "3D-SHORE needs diffusion-weighted data on at least 2 b-value shells; this series has " + what +
+                        ". With a single shell the radial decay of the signal is not determined by the data: a fit "
+                        "would match the measurements, but its propagator-derived scalars (RTOP, RTAP, RTPP, MSD, "
+                        "QIV) and the signals it predicts at other b-values would reflect the regularization rather "
+                        "than the data. For single-shell data use single-shell three-tissue CSD (cs-ss3t-full, "
+                        "cs_dmri.ss3t_pipeline). To fit anyway for orientation information only, allow single-shell "
+                        "data (--allow-single-shell, or allow_single_shell=True in Python)."};
+            warn.msgs.push_back("the data have " + what +
+                                ": only the orientation information of this fit is meaningful, and "
+                                "propagator-derived scalars are not computed");
+        }
+        const int K = n_coeffs(sp.radial_order);
+        out->n_coeffs = K;
+        if (N < K && sp.regularization != SHOREFIT_REG_L1)
+            warn.msgs.push_back(std::to_string(N) + " measurements for " + std::to_string(K) +
+                                " coefficients (radial order " + std::to_string(sp.radial_order) +
+                                "): without L1 regularization the fit is underdetermined and its fit statistics "
+                                "are not meaningful; use a lower radial order or L1");
+        if (sp.regularization == SHOREFIT_REG_L1 &&
+            (sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC || sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED))
+            throw Error{SHOREFIT_E_UNSUPPORTED,
+                        "alpha path strategies (path-bic, l2-anchored) are not implemented on this device yet; "
+                        "use alpha_mode 'fixed' or 'alpha-ratio'"};
+
+        // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
+        const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
+        std::vector<uint8_t> mask_h;
+        const uint8_t* mask = vol->mask;
+        if (mask && vol->on_device) {
+            mask_h.resize(nvox_total);
+            if (nvox_total) SF_HIP(hipMemcpy(mask_h.data(), mask, nvox_total, hipMemcpyDeviceToHost));
+            mask = mask_h.data();
+        }
+        std::vector<int64_t> vox;
+        vox.reserve(nvox_total);
+        for (int64_t i = 0; i < nvox_total; ++i)
+            if (!mask || mask[i]) vox.push_back(i);
+        const int64_t V = int64_t(vox.size());
+        out->n_voxels = V;
+
+        // --- device buffers -------------------------------------------------
+        const size_t coef_bytes = sizeof(float) * size_t(nvox_total) * K;
+        DevBuf<float> d_data, d_coef, d_r2, d_res, d_alpha, d_bic, d_rssl2;
+        DevBuf<uint32_t> d_it;
+        DevBuf<uint8_t> d_kind;
+        DevBuf<int64_t> d_vox(V);
+        const bool dev = vol->on_device != 0;
+        const float* data = vol->data;
+        if (!dev) {
+            d_data.alloc(size_t(nvox_total) * N);
+            if (nvox_total)
+                SF_HIP(hipMemcpyAsync(d_data.p, vol->data, sizeof(float) * nvox_total * N, hipMemcpyHostToDevice, s));
+            data = d_data.p;
+        }
+        if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
+        auto out_buf = [&](auto* user, auto& buf, size_t count) {
+            using T = std::remove_pointer_t<decltype(user)>;
+            if (!user) return static_cast<T*>(nullptr);
+            if (dev) return user;
+            buf.alloc(count);
+            return buf.p;
+        };
+        const bool l1 = sp.regularization == SHOREFIT_REG_L1;
+        float* coef = out_buf(out->coefficients, d_coef, size_t(nvox_total) * K);
+        float* r2 = out_buf(out->r2, d_r2, nvox_total);
+        float* res = out_buf(out->residual_l2, d_res, nvox_total);
+        uint32_t* its = out_buf(out->iterations, d_it, nvox_total);
+        uint8_t* kind = out_buf(out->regularization_kind, d_kind, nvox_total);
+        float* amap = l1 ? out_buf(out->alpha, d_alpha, nvox_total) : nullptr;
+        float* bic = l1 ? out_buf(out->bic, d_bic, nvox_total) : nullptr;
+        DevBuf<float> d_alpha_tmp;
+        const bool per_voxel_alpha = l1 && sp.alpha_mode != SHOREFIT_ALPHA_FIXED;
+        if (per_voxel_alpha && !amap) {
+            d_alpha_tmp.alloc(nvox_total);
+            amap = d_alpha_tmp.p;
+        }
+        // zeros outside the mask (src/fit.rs:89-101)
+        if (nvox_total) {
+            SF_HIP(hipMemsetAsync(coef, 0, coef_bytes, s));
+            if (r2) SF_HIP(hipMemsetAsync(r2, 0, sizeof(float) * nvox_total, s));
+            if (res) SF_HIP(hipMemsetAsync(res, 0, sizeof(float) * nvox_total, s));
+            if (its) SF_HIP(hipMemsetAsync(its, 0, sizeof(uint32_t) * nvox_total, s));
+            if (kind) SF_HIP(hipMemsetAsync(kind, 0, nvox_total, s));
+            if (amap) SF_HIP(hipMemsetAsync(amap, 0, sizeof(float) * nvox_total, s));
+            if (bic) SF_HIP(hipMemsetAsync(bic, 0, sizeof(float) * nvox_total, s));
+        }
+        if (out->rss_l2 && dev && nvox_total) SF_HIP(hipMemsetAsync(out->rss_l2, 0, sizeof(float) * nvox_total, s));
+
+        // --- design + solver set-up (src/fit.rs:718-759) -------------------
+        Design d;
+        build_design(d, *grads, big, small, sp.radial_order, sp.zeta, s);
+        hipEvent_t e0, e1;
+        SF_HIP(hipEventCreate(&e0));
+        SF_HIP(hipEventCreate(&e1));
+        if (l1) {
+            gram_and_lipschitz(d, s);
+            out->lipschitz = d.L;
+            FistaOps ops;
+            pack_fista(d, ops, sp.precision, s);
+            FistaLaunch L{};
+            L.data = data;
+            L.vox = d_vox.p;
+            L.n_vox = V;
+            L.N = d.N;
+            L.sig_stride = N;
+            L.K = d.K;
+            L.Kp = d.Kp;
+            L.Np = d.Np;
+            L.Mfrag = ops.M.p;
+            L.ATfrag = ops.AT.p;
+            L.Afrag = ops.A.p;
+            L.step = float(ops.step);
+            L.thr = float(ops.step * sp.alpha);
+            L.ratio = sp.alpha_mode == SHOREFIT_ALPHA_RATIO ? float(sp.alpha_ratio) : 0.f;
+            L.alpha = sp.alpha;
+            L.max_iter = sp.max_iter;
+            L.tol = float(sp.tol);
+            L.nonneg = sp.non_negative;
+            L.precision = sp.precision;
+            L.refresh = std::max(1, sp.refresh_every);
+            L.coef = coef;
+            L.resid = res;
+            L.r2 = r2;
+            L.iters = its;
+            L.kind = kind;
+            L.alpha_map = amap;
+            L.bic = bic;
+            SF_HIP(hipEventRecord(e0, s));
+            launch_fista(L, s);
+            SF_HIP(hipEventRecord(e1, s));
+        } else {
+            d.G.alloc(size_t(d.K) * d.K);
+            launch_gram(d.A.p, d.N, d.K, d.G.p, s);
+            std::vector<double> dn, dl;
+            regularization_diagonals(sp.radial_order, dn, dl);
+            TikOps ops;
+            pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, ops, s);
+            TikhonovLaunch L{};
+            L.data = data;
+            L.vox = d_vox.p;
+            L.n_vox = V;
+            L.N = d.N;
+            L.sig_stride = N;
+            L.K = d.K;
+            L.Kp = d.Kp;
+            L.Np = d.Np;
+            L.Pfrag = ops.P.p;
+            L.Afrag = ops.A.p;
+            L.coef = coef;
+            L.resid = res;
+            L.r2 = r2;
+            L.iters = its;
+            L.kind = kind;
+            SF_HIP(hipEventRecord(e0, s));
+            launch_tikhonov(L, s);
+            SF_HIP(hipEventRecord(e1, s));
+        }
+        SF_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        out->fit_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+
+        // --- alpha distribution (src/fit.rs:334-352) ------------------------
+        out->has_alpha_distribution = 0;
+        if (per_voxel_alpha) {
+            std::vector<float> a_all(nvox_total);
+            if (nvox_total)
+                SF_HIP(hipMemcpy(a_all.data(), amap, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+            std::vector<double> seen;
+            seen.reserve(V);
+            for (int64_t i : vox) seen.push_back(double(a_all[i]));
+            std::sort(seen.begin(), seen.end());
+            out->has_alpha_distribution = 1;
+            if (seen.empty()) {
+                out->alpha_distribution[0] = out->alpha_distribution[1] = out->alpha_distribution[2] = 0.0;
+            } else {
+                auto pct = [&](double q) {
+                    size_t i = size_t(std::llround(double(seen.size() - 1) * q));
+                    return seen[std::min(i, seen.size() - 1)];
+                };
+                out->alpha_distribution[0] = pct(0.5);
+                out->alpha_distribution[1] = pct(0.10);
+                out->alpha_distribution[2] = pct(0.90);
+            }
+        }
+
+        // --- results back to the caller -------------------------------------
+        if (!dev && nvox_total) {
+            SF_HIP(hipMemcpy(out->coefficients, coef, coef_bytes, hipMemcpyDeviceToHost));
+            if (out->r2) SF_HIP(hipMemcpy(out->r2, r2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+            if (out->residual_l2)
+                SF_HIP(hipMemcpy(out->residual_l2, res, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+            if (out->iterations)
+                SF_HIP(hipMemcpy(out->iterations, its, sizeof(uint32_t) * nvox_total, hipMemcpyDeviceToHost));
+            if (out->regularization_kind)
+                SF_HIP(hipMemcpy(out->regularization_kind, kind, nvox_total, hipMemcpyDeviceToHost));
+            if (l1 && out->alpha) SF_HIP(hipMemcpy(out->alpha, amap, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+            if (l1 && out->bic) SF_HIP(hipMemcpy(out->bic, bic, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+        }
+        warn.to(out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Internal declarations shared by the shorefit translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "shorefit.h"
+
+namespace shorefit {
+
+// Thread-local last-error string behind shorefit_last_error().
+void set_error(const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define SF_HIP(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            throw ::shorefit::Error{SHOREFIT_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+
+// Owning device buffer (freed on scope exit; errors on free are ignored).
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) SF_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// ---------------------------------------------------------------- basis ----
+int n_coeffs(uint32_t radial_order);
+
+// Device FP64 design matrix (N x K row-major) from q-magnitudes and bvecs.
+void launch_design_matrix(const double* d_q, const double* d_bvecs, int N, uint32_t radial_order, double zeta,
+                          double* d_A, hipStream_t s);
+
+// ------------------------------------------------------------ FP64 setup ----
+// G = A^T A (K x K), A row-major N x K.
+void launch_gram(const double* d_A, int N, int K, double* d_G, hipStream_t s);
+// Largest eigenvalue of the symmetric K x K matrix G by power iteration
+// (Rayleigh quotient), written to d_lambda[0].
+void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStream_t s);
+// In-place lower Cholesky of Greg (K x K, upper triangle zeroed); d_status[0]
+// = 0 on success, j+1 when pivot j is not positive.
+void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s);
+
+// ------------------------------------------------------------- packing ----
+// Fragment-major FP32 operands of the 16x16x4 f32 MFMA (layout described in
+// fista.hip). Padded rows/cols are zero.
+//   M  = I - c * G            (Kp x Kp), c = step / N
+//   AT = c * A^T              (Kp x Np)
+//   A                          (Np x Kp)
+void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hipStream_t s);
+void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, bool transpose, double scale,
+                          int rows_p, int cols_p, float* d_frag, hipStream_t s);
+// Same operand in FP16 / BF16 16x16x32 fragment order (mixed kernel).
+void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s);
+
+// ---------------------------------------------------------------- fits ----
+struct FistaLaunch {
+    // signals: row v of the fit is data[vox[v] * sig_stride ...] (N floats);
+    // vox == nullptr means row v itself.
+    const float* data;
+    const int64_t* vox;
+    int64_t n_vox;
+    int N;
+    int64_t sig_stride;
+    int K, Kp, Np;
+    const float* Mfrag;
+    const float* ATfrag;
+    const float* Afrag;
+    const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
+    float thr;              // step * alpha (fixed alpha)
+    float step;             // 1 / L
+    float ratio;            // alpha-ratio strategy: alpha = ratio * alpha_max (0 = fixed)
+    double alpha;           // fixed alpha (for maps / BIC)
+    uint32_t max_iter;
+    float tol;
+    int nonneg;
+    int precision;
+    int refresh;
+    // outputs, indexed by vox[v] (or v when vox == nullptr)
+    float* coef;            // [.. x K]
+    float* resid;           // optional
+    float* r2;              // optional
+    uint32_t* iters;        // optional
+    uint8_t* converged;     // optional
+    uint8_t* kind;          // optional
+    float* alpha_map;       // optional
+    float* bic;             // optional
+    // persistent-kernel work counter (mixed precision)
+    unsigned int* counter;
+};
+void launch_fista(const FistaLaunch& L, hipStream_t s);
+
+struct TikhonovLaunch {
+    const float* data;
+    const int64_t* vox;
+    int64_t n_vox;
+    int N;
+    int64_t sig_stride;
+    int K, Kp, Np;
+    const float* Pfrag;  // (G + Lambda)^-1 A^T, Kp x Np fragment-major
+    const float* Afrag;
+    float* coef;
+    float* resid;
+    float* r2;
+    uint32_t* iters;
+    uint8_t* kind;
+    float* rss_l2;        // optional: rss of this fit (L2-anchored reference)
+};
+void launch_tikhonov(const TikhonovLaunch& L, hipStream_t s);
+
+}  // namespace shorefit

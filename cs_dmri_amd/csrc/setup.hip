@@ -1,0 +1,348 @@
+// FP64 set-up kernels: SHORE design matrix, Gram matrix, Lipschitz constant
+// (power iteration), Cholesky factor, and the packing of FP32 / FP16 / BF16
+// MFMA operands. Everything here runs once per fit (or once per design).
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "common.h"
+
+namespace shorefit {
+
+int n_coeffs(uint32_t radial_order) {
+    // src/basis/shore.rs:29-44: for n in 0..=R, even l <= n, 2l+1 slots.
+    int k = 0;
+    for (uint32_t n = 0; n <= radial_order; ++n)
+        for (uint32_t l = 0; l <= n; l += 2) k += int(2 * l + 1);
+    return k;
+}
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+__device__ double factorial_d(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= double(i);
+    return f;
+}
+
+// Gamma(n + 3/2) = (2n+1)!! / 2^(n+1) * sqrt(pi), exact product form.
+__device__ double gamma_n_three_halves(int n) {
+    double g = sqrt(kPi) * 0.5;  // Gamma(3/2)
+    for (int i = 1; i <= n; ++i) g *= (double(i) + 0.5);
+    return g;
+}
+
+// Generalised Laguerre L_n^a(x) by its three-term recurrence.
+__device__ double laguerre_d(int n, double a, double x) {
+    double lm1 = 1.0;
+    if (n == 0) return lm1;
+    double l = 1.0 + a - x;
+    for (int k = 1; k < n; ++k) {
+        const double lp1 = ((2.0 * k + 1.0 + a - x) * l - (k + a) * lm1) / (k + 1.0);
+        lm1 = l;
+        l = lp1;
+    }
+    return l;
+}
+
+// Associated Legendre P_l^m(x), Condon-Shortley phase, upward recurrence.
+__device__ double legendre_d(int l, int m, double x) {
+    double pmm = 1.0;
+    if (m > 0) {
+        const double s = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+        double f = 1.0;
+        for (int i = 0; i < m; ++i) {
+            pmm *= -f * s;
+            f += 2.0;
+        }
+    }
+    if (l == m) return pmm;
+    double p1 = x * (2.0 * m + 1.0) * pmm;
+    if (l == m + 1) return p1;
+    double pl = 0.0;
+    for (int ll = m + 2; ll <= l; ++ll) {
+        pl = ((2.0 * ll - 1.0) * x * p1 - (ll + m - 1.0) * pmm) / double(ll - m);
+        pmm = p1;
+        p1 = pl;
+    }
+    return pl;
+}
+
+// One entry of the SHORE design matrix. Column col maps to (n, l, slot)
+// in the order of src/basis/shore.rs:29-44; the angular factor follows the
+// BrainSuite real-SH block of src/basis/shore.rs:76-115 and the radial
+// factor src/basis/shore.rs:56-70.
+__global__ void design_matrix_kernel(const double* __restrict__ q, const double* __restrict__ bvecs, int N, int R,
+                                     double zeta, int K, double* __restrict__ A) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(N) * K) return;
+    const int i = int(t / K), col = int(t % K);
+    int n = 0, l = 0, slot = 0, c = 0;
+    for (int nn = 0; nn <= R; ++nn) {
+        bool found = false;
+        for (int ll = 0; ll <= nn; ll += 2) {
+            if (col < c + 2 * ll + 1) {
+                n = nn;
+                l = ll;
+                slot = col - c;
+                found = true;
+                break;
+            }
+            c += 2 * ll + 1;
+        }
+        if (found) break;
+    }
+    const double qq = q[i];
+    const double x = bvecs[3 * i] * qq, y = bvecs[3 * i + 1] * qq, z = bvecs[3 * i + 2] * qq;
+    const double r = sqrt(x * x + y * y + z * z);
+    double theta = atan2(y, x);
+    if (isnan(theta)) theta = 0.0;
+    const double phi = r > 0.0 ? acos(fmin(1.0, fmax(-1.0, z / r))) : 0.0;
+
+    const double u = r * r / zeta;
+    const double kap = sqrt(2.0 * factorial_d(n - l) / (pow(zeta, 1.5) * gamma_n_three_halves(n)));
+    const double radial = kap * laguerre_d(n - l, l + 0.5, u) * exp(-0.5 * u) * pow(u, 0.5 * l);
+
+    double ang;
+    if (l == 0) {
+        ang = 1.0 / sqrt(4.0 * kPi);
+    } else {
+        const int k = slot < l ? l - slot : slot - l;  // |order| of the slot
+        const double norm = sqrt((2.0 * l + 1.0) / (4.0 * kPi) * factorial_d(l - k) / factorial_d(l + k));
+        const double p = norm * legendre_d(l, k, cos(phi));
+        if (slot < l)
+            ang = sqrt(2.0) * p * cos(k * theta);
+        else if (slot == l)
+            ang = p;
+        else
+            ang = sqrt(2.0) * ((k % 2 == 1) ? 1.0 : -1.0) * p * sin(k * theta);
+    }
+    A[t] = radial * ang;
+}
+
+__global__ void gram_kernel(const double* __restrict__ A, int N, int K, double* __restrict__ G) {
+    const int p = blockIdx.y, q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= K) return;
+    double acc = 0.0;
+    for (int i = 0; i < N; ++i) acc += A[size_t(i) * K + p] * A[size_t(i) * K + q];
+    G[size_t(p) * K + q] = acc;
+}
+
+__device__ double block_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) s += red[i];
+    return s;
+}
+
+// Power iteration on the symmetric PSD Gram matrix: lambda = v^T G v with
+// v <- G v / |G v|, stopped once the Rayleigh quotient has not moved by more
+// than 1e-16 relative for 16 consecutive sweeps (or after 200000 sweeps).
+// The dominant-eigenvalue gap of SHORE Gram matrices is ~0.86-0.94, so this
+// settles in a few hundred sweeps.
+__global__ void __launch_bounds__(256) power_iteration_kernel(const double* __restrict__ G, int K,
+                                                              double* __restrict__ out) {
+    extern __shared__ double sm[];
+    double* v = sm;
+    double* w = sm + K;
+    __shared__ double red[8];
+    for (int i = threadIdx.x; i < K; i += blockDim.x) v[i] = 1.0 / sqrt(double(K)) * (1.0 + 1e-3 * i);
+    __syncthreads();
+    double lam = 0.0;
+    int stable = 0;
+    for (int it = 0; it < 200000 && stable < 16; ++it) {
+        double part = 0.0, nrm = 0.0, vpart = 0.0;
+        for (int p = threadIdx.x; p < K; p += blockDim.x) {
+            double acc = 0.0;
+            for (int q = 0; q < K; ++q) acc += G[size_t(p) * K + q] * v[q];
+            w[p] = acc;
+            part += acc * v[p];
+            nrm += acc * acc;
+            vpart += v[p] * v[p];
+        }
+        const double vv = block_sum(vpart, red);
+        const double num = block_sum(part, red);
+        const double ww = block_sum(nrm, red);
+        const double lam_new = num / vv;
+        stable = (fabs(lam_new - lam) <= 1e-16 * fabs(lam_new)) ? stable + 1 : 0;
+        lam = lam_new;
+        const double inv = ww > 0.0 ? 1.0 / sqrt(ww) : 0.0;
+        __syncthreads();
+        for (int p = threadIdx.x; p < K; p += blockDim.x) v[p] = w[p] * inv;
+        __syncthreads();
+        if (ww == 0.0) break;
+    }
+    if (threadIdx.x == 0) out[0] = lam;
+}
+
+// Right-looking Cholesky in one workgroup (K <= a few hundred).
+__global__ void __launch_bounds__(256) cholesky_kernel(double* __restrict__ G, int K, int* __restrict__ status) {
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < K; ++j) {
+        __syncthreads();
+        const double d = G[size_t(j) * K + j];
+        if (!(d > 0.0)) {
+            if (threadIdx.x == 0) bad = j + 1;
+            __syncthreads();
+            break;
+        }
+        const double s = sqrt(d);
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < K; i += blockDim.x) G[size_t(i) * K + j] /= s;
+        __syncthreads();
+        if (threadIdx.x == 0) G[size_t(j) * K + j] = s;
+        // trailing update of the lower triangle
+        const int m = K - j - 1;
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+            const int a = j + 1 + t / m, b = j + 1 + t % m;
+            if (b <= a) G[size_t(a) * K + b] -= G[size_t(a) * K + j] * G[size_t(b) * K + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
+        const int a = t / K, b = t % K;
+        if (b > a) G[t] = 0.0;
+    }
+    if (threadIdx.x == 0) status[0] = bad;
+}
+
+// Fragment-major operand of __builtin_amdgcn_mfma_f32_16x16x4f32: for
+// 16x16 block (rb, cb) and lane l, four floats X[16rb + l%16][16cb + 4(l/16) + r].
+__global__ void pack_rowblock_kernel(const double* __restrict__ src, int rows, int cols, int ld, int transpose,
+                                     double scale, int rows_p, int cols_p, float* __restrict__ frag) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int CB = cols_p / 16;
+    const int64_t total = int64_t(rows_p) * cols_p;
+    if (t >= total) return;
+    const int r = int(t & 3), l = int((t >> 2) & 63);
+    const int64_t blk = t >> 8;
+    const int rb = int(blk / CB), cb = int(blk % CB);
+    const int i = 16 * rb + (l & 15), j = 16 * cb + 4 * (l >> 4) + r;
+    double v = 0.0;
+    if (i < rows && j < cols) v = scale * (transpose ? src[size_t(j) * ld + i] : src[size_t(i) * ld + j]);
+    frag[t] = float(v);
+}
+
+// M = I - c G in the same fragment order, zero outside K x K.
+__global__ void pack_M_kernel(const double* __restrict__ G, int K, int Kp, double c, float* __restrict__ frag) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int MB = Kp / 16;
+    if (t >= int64_t(Kp) * Kp) return;
+    const int r = int(t & 3), l = int((t >> 2) & 63);
+    const int64_t blk = t >> 8;
+    const int rb = int(blk / MB), cb = int(blk % MB);
+    const int i = 16 * rb + (l & 15), j = 16 * cb + 4 * (l >> 4) + r;
+    double v = 0.0;
+    if (i < K && j < K) v = (i == j ? 1.0 : 0.0) - c * G[size_t(i) * K + j];
+    frag[t] = float(v);
+}
+
+// M = I - c G as the A operand of the 16x16x32 FP16/BF16 MFMA: for output
+// block mb, k-chunk kc (32 rows of Y, i.e. 16-row blocks 2kc and 2kc+1) and
+// lane l = 16h + i, eight values M[16mb + i][p(v)], v = 0..7, where
+// p(v) = 32kc + 16(v/4) + 4h + v%4 is the row of Y that lane (h, j) of the
+// accumulator layout holds in register v%4 of block 2kc + v/4.
+__global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, double c, int bf16,
+                                   uint16_t* __restrict__ frag) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int KC = Kp / 32;
+    if (t >= int64_t(Kp) * Kp) return;
+    const int v = int(t & 7), l = int((t >> 3) & 63);
+    const int64_t blk = t >> 9;
+    const int mb = int(blk / KC), kc = int(blk % KC);
+    const int h = l >> 4, i = 16 * mb + (l & 15);
+    const int p = 32 * kc + 16 * (v >> 2) + 4 * h + (v & 3);
+    double x = 0.0;
+    if (i < K && p < K) x = (i == p ? 1.0 : 0.0) - c * G[size_t(i) * K + p];
+    if (bf16) {
+        __hip_bfloat16 b = __float2bfloat16(float(x));
+        frag[t] = *reinterpret_cast<uint16_t*>(&b);
+    } else {
+        __half hv = __float2half(float(x));
+        frag[t] = *reinterpret_cast<uint16_t*>(&hv);
+    }
+}
+
+// Column n of P = (L L^T)^-1 A^T is the solution for right-hand side row n of
+// A (forward then backward substitution), computed in place in P's column
+// (P is K x N row-major, so threads of consecutive n touch consecutive words).
+__global__ void chol_solve_columns_kernel(const double* __restrict__ Lf, int K, const double* __restrict__ A, int N,
+                                          double* __restrict__ P) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    for (int i = 0; i < K; ++i) {
+        double v = A[size_t(n) * K + i];
+        for (int k = 0; k < i; ++k) v -= Lf[size_t(i) * K + k] * P[size_t(k) * N + n];
+        P[size_t(i) * N + n] = v / Lf[size_t(i) * K + i];
+    }
+    for (int i = K - 1; i >= 0; --i) {
+        double v = P[size_t(i) * N + n];
+        for (int k = i + 1; k < K; ++k) v -= Lf[size_t(k) * K + i] * P[size_t(k) * N + n];
+        P[size_t(i) * N + n] = v / Lf[size_t(i) * K + i];
+    }
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_design_matrix(const double* d_q, const double* d_bvecs, int N, uint32_t R, double zeta, double* d_A,
+                          hipStream_t s) {
+    const int K = n_coeffs(R);
+    const int64_t total = int64_t(N) * K;
+    hipLaunchKernelGGL(design_matrix_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_q, d_bvecs, N,
+                       int(R), zeta, K, d_A);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_gram(const double* d_A, int N, int K, double* d_G, hipStream_t s) {
+    hipLaunchKernelGGL(gram_kernel, dim3(blocks_for(K, 64), K), dim3(64), 0, s, d_A, N, K, d_G);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStream_t s) {
+    hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(256), size_t(2 * K) * sizeof(double), s, d_G, K,
+                       d_lambda);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s) {
+    hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(256), 0, s, d_G, K, d_status);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hipStream_t s) {
+    const int64_t total = int64_t(Kp) * Kp;
+    hipLaunchKernelGGL(pack_M_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_G, K, Kp, c, d_frag);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, bool transpose, double scale,
+                          int rows_p, int cols_p, float* d_frag, hipStream_t s) {
+    const int64_t total = int64_t(rows_p) * cols_p;
+    hipLaunchKernelGGL(pack_rowblock_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_src, rows, cols,
+                       src_ld, int(transpose), scale, rows_p, cols_p, d_frag);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s) {
+    const int64_t total = int64_t(Kp) * Kp;
+    hipLaunchKernelGGL(pack_M_lowp_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_G, K, Kp, c,
+                       int(bf16), d_frag);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_chol_solve_columns(const double* d_L, int K, const double* d_A, int N, double* d_P, hipStream_t s) {
+    hipLaunchKernelGGL(chol_solve_columns_kernel, dim3(blocks_for(N, 64)), dim3(64), 0, s, d_L, K, d_A, N, d_P);
+    SF_HIP(hipGetLastError());
+}
+
+}  // namespace shorefit

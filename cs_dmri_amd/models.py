@@ -1,0 +1,212 @@
+"""``shore_fit``: the SHORE-fit entry point with the keyword set of the
+reference's ``_cs_dmri.shore_fit`` (python/src/models.rs:70-198), computed by
+``libshorefit.so`` on the GPU, plus the solver-level helpers the reference's
+tests drive (FistaSolver / TikhonovSolver / ShoreBasis::design_matrix).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+TORTOISE_DEFAULT_GMAX = 80e-3
+
+
+def n_coeffs(radial_order: int) -> int:
+    k = L.lib().shorefit_n_coeffs(int(radial_order))
+    if k < 0:
+        raise ValueError(f"radial_order must be a positive even number, got {radial_order}")
+    return k
+
+
+def design_matrix(bvals, bvecs, *, radial_order=6, zeta=700.0, big_delta=None, small_delta=None,
+                  b0_threshold=50.0) -> np.ndarray:
+    """ShoreBasis::design_matrix (src/basis/shore.rs:122-178), FP64, evaluated on the device."""
+    g, keep = L.gradients(bvals, bvecs, big_delta, small_delta, b0_threshold)
+    A = np.zeros((g.n_grads, n_coeffs(radial_order)), dtype=np.float64)
+    L.check(L.lib().shorefit_design_matrix(C.byref(g), int(radial_order), float(zeta), L.ptr(A, L._pd)))
+    return A
+
+
+def fista_batch(A, signals, alpha, *, max_iter=1000, tol=1e-6, non_negative=False, precision="fp32",
+                refresh_every=10):
+    """FistaSolver::fit (src/solver/fista.rs:38-150) for every row of ``signals`` against design ``A``.
+
+    Returns ``(coef (V,K) f32, dict(iterations, converged, residual_l2, lipschitz))``.
+    """
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    S = np.ascontiguousarray(np.atleast_2d(signals), dtype=np.float32)
+    N, K = A.shape
+    if S.shape[1] != N:
+        raise ValueError(f"signals have {S.shape[1]} samples but the design has {N} rows")
+    V = S.shape[0]
+    coef = np.zeros((V, K), np.float32)
+    its = np.zeros(V, np.uint32)
+    conv = np.zeros(V, np.uint8)
+    res = np.zeros(V, np.float32)
+    lip = C.c_double()
+    L.check(L.lib().shorefit_fista_batch(
+        L.ptr(A, L._pd), N, K, L.ptr(S, L._pf), V, float(alpha), int(max_iter), float(tol), int(non_negative),
+        L.PRECISION[precision], int(refresh_every), L.ptr(coef, L._pf), its.ctypes.data_as(C.POINTER(C.c_uint32)),
+        conv.ctypes.data_as(C.POINTER(C.c_uint8)), L.ptr(res, L._pf), C.byref(lip)))
+    return coef, {"iterations": its, "converged": conv.astype(bool), "residual_l2": res, "lipschitz": lip.value}
+
+
+def tikhonov_batch(A, signals, d_n=None, d_l=None, lambda_n=0.0, lambda_l=0.0):
+    """TikhonovSolver::fit (src/solver/tikhonov.rs:26-76) for every row of ``signals``."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    S = np.ascontiguousarray(np.atleast_2d(signals), dtype=np.float32)
+    N, K = A.shape
+    V = S.shape[0]
+    dn = None if d_n is None else np.ascontiguousarray(d_n, dtype=np.float64)
+    dl = None if d_l is None else np.ascontiguousarray(d_l, dtype=np.float64)
+    coef = np.zeros((V, K), np.float32)
+    res = np.zeros(V, np.float32)
+    L.check(L.lib().shorefit_tikhonov_batch(L.ptr(A, L._pd), N, K, L.ptr(dn, L._pd), L.ptr(dl, L._pd),
+                                            float(lambda_n), float(lambda_l), L.ptr(S, L._pf), V,
+                                            L.ptr(coef, L._pf), L.ptr(res, L._pf)))
+    return coef, res
+
+
+def _spec(radial_order, zeta, regularization, alpha_mode, alpha, alpha_ratio, path_n_alphas, path_eps, slack,
+          max_iter, tol, non_negative, lambda_n, lambda_l, diagnostics, allow_single_shell, precision,
+          refresh_every) -> L.ShorefitSpec:
+    if regularization not in L.REG:
+        raise ValueError(f"regularization must be l1, l2 or nonneg, got {regularization!r}")
+    if regularization == "l1" and alpha_mode not in L.ALPHA_MODE:
+        raise ValueError(f"alpha_mode must be fixed, alpha-ratio, path-bic or l2-anchored, got {alpha_mode!r}")
+    if precision not in L.PRECISION:
+        raise ValueError(f"precision must be one of {tuple(L.PRECISION)}, got {precision!r}")
+    s = L.ShorefitSpec()
+    L.lib().shorefit_spec_default(C.byref(s))
+    s.radial_order = int(radial_order)
+    s.zeta = float(zeta)
+    s.regularization = L.REG[regularization]
+    s.alpha_mode = L.ALPHA_MODE.get(alpha_mode, 0)
+    s.alpha = float(alpha)
+    s.alpha_ratio = float(alpha_ratio)
+    s.path_n_alphas = int(path_n_alphas)
+    s.path_eps = 0.0 if path_eps is None else float(path_eps)
+    s.slack = float(slack)
+    s.max_iter = int(max_iter)
+    s.tol = float(tol)
+    s.non_negative = int(bool(non_negative))
+    s.lambda_n = float(lambda_n)
+    s.lambda_l = float(lambda_l)
+    s.compute_diagnostics = int(bool(diagnostics))
+    s.allow_single_shell = int(bool(allow_single_shell))
+    s.precision = L.PRECISION[precision]
+    s.refresh_every = int(refresh_every)
+    return s
+
+
+def _sidecar(out: L.ShorefitOutput, spec: L.ShorefitSpec, regularization, alpha_mode, alpha, alpha_ratio,
+             path_n_alphas, path_eps, slack, max_iter, tol, non_negative, lambda_n, lambda_l, bvec_frame,
+             n_coeffs_: int, radial_order, zeta, alpha_distribution) -> dict:
+    """SidecarMetadata dict (src/io/coeffs.rs:28-88), as the binding returns it."""
+    if regularization == "l1":
+        eps = path_eps if path_eps is not None else (1e-4 if alpha_mode == "l2-anchored" else 1e-3)
+        strat = {
+            "fixed": {"kind": "fixed", "alpha": alpha},
+            "alpha-ratio": {"kind": "alpha-max-ratio", "ratio": alpha_ratio},
+            "path-bic": {"kind": "path-bic", "path": {"n": path_n_alphas, "eps": eps}},
+            "l2-anchored": {"kind": "path-l2-anchored", "path": {"n": path_n_alphas, "eps": eps}, "slack": slack},
+        }[alpha_mode]
+        if alpha_mode == "fixed":
+            chosen = {"kind": "global", "alpha": alpha}
+        else:
+            med, p10, p90 = alpha_distribution
+            chosen = {"kind": "per-voxel", "median": med, "p10": p10, "p90": p90}
+        solver = {"kind": "fista", "alpha_strategy": strat, "chosen_alpha": chosen, "non_negative": non_negative,
+                  "max_iter": max_iter, "tol": tol}
+    else:
+        solver = {"kind": "tikhonov", "lambda_n": lambda_n, "lambda_l": lambda_l}
+    return {
+        "basis": {"name": "shore", "radial_order": radial_order, "zeta": zeta},
+        "big_delta_seconds": out.big_delta,
+        "small_delta_seconds": out.small_delta,
+        "tau_seconds": out.tau,
+        "delta_source": "estimated_tortoise" if out.delta_source else "user_supplied",
+        "gmax_tesla_per_meter": TORTOISE_DEFAULT_GMAX,
+        "solver": solver,
+        "n_coefficients": n_coeffs_,
+        "bvec_frame": bvec_frame,
+        "dwi_shells": [float(out.dwi_shells[i]) for i in range(min(out.n_shells, 32))],
+        "provenance": None,
+    }
+
+
+def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_threshold=50.0,
+              bvec_frame="image-axis", radial_order=6, zeta=700.0, regularization="l1", alpha_mode="l2-anchored",
+              alpha=1.0, alpha_ratio=1e-3, path_n_alphas=20, path_eps=None, slack=0.05, max_iter=1000, tol=1e-6,
+              non_negative=False, lambda_n=1e-8, lambda_l=1e-8, nonneg_max_iter=200, nonneg_tol=1e-9,
+              nonneg_epsilon=1e-10, diagnostics=False, allow_single_shell=False, n_threads=None,
+              precision="fp32", refresh_every=10) -> dict:
+    """Drop-in for ``_cs_dmri.shore_fit`` (python/src/models.rs:70-198).
+
+    Same keywords, defaults, validation messages (``ValueError``) and returned
+    keys: ``coefficients`` (X,Y,Z,K) float32, ``warnings``, ``sidecar``,
+    ``alpha_distribution`` and, with ``diagnostics``, ``rmse``, ``r2``,
+    ``residual_l2``, ``iterations``, ``regularization_kind`` (+ ``alpha``,
+    ``bic``, ``rss_l2`` where the solver has them). ``nonneg_*`` and
+    ``n_threads`` are accepted for signature parity; ``precision`` and
+    ``refresh_every`` select the device arithmetic.
+    """
+    if bvec_frame not in ("image-axis", "world-ras"):
+        raise ValueError(f"bvec_frame must be 'image-axis' or 'world-ras', got {bvec_frame!r}")
+    spec = _spec(radial_order, zeta, regularization, alpha_mode, alpha, alpha_ratio, path_n_alphas, path_eps,
+                 slack, max_iter, tol, non_negative, lambda_n, lambda_l, diagnostics, allow_single_shell,
+                 precision, refresh_every)
+    vol = np.asarray(data)
+    if vol.ndim != 4:
+        raise ValueError(f"data must be 4-D (X, Y, Z, N), got shape {vol.shape}")
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    g, keep = L.gradients(bvals, bvecs, big_delta, small_delta, b0_threshold)
+    if vol.shape[3] != g.n_grads:
+        raise ValueError(f"data has {vol.shape[3]} volumes but the gradient table has {g.n_grads}")
+    m = np.asarray(mask)
+    if m.shape != vol.shape[:3]:
+        raise ValueError(f"mask shape {m.shape} does not match data spatial shape {vol.shape[:3]}")
+    m = np.ascontiguousarray(m.astype(np.uint8))
+    nx, ny, nz, _ = vol.shape
+    K = n_coeffs(radial_order) if radial_order % 2 == 0 and radial_order > 0 else 0
+    shp = (nx, ny, nz)
+    coefs = np.zeros(shp + (K,), np.float32)
+    diag = bool(diagnostics)
+    l1 = regularization == "l1"
+    r2 = np.zeros(shp, np.float32) if diag else None
+    res = np.zeros(shp, np.float32) if diag else None
+    its = np.zeros(shp, np.uint32) if diag else None
+    kind = np.zeros(shp, np.uint8) if diag else None
+    amap = np.zeros(shp, np.float32) if (diag and l1) else None
+    bic = np.zeros(shp, np.float32) if (diag and l1) else None
+    out = L.ShorefitOutput()
+    out.coefficients = L.ptr(coefs).value
+    for name, arr in (("r2", r2), ("residual_l2", res), ("iterations", its), ("regularization_kind", kind),
+                      ("alpha", amap), ("bic", bic)):
+        setattr(out, name, None if arr is None else arr.ctypes.data)
+    v = L.ShorefitVolume(vol.ctypes.data, m.ctypes.data, nx, ny, nz, 0, 0)
+    L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), None))
+    dist = tuple(out.alpha_distribution) if out.has_alpha_distribution else None
+    result = {
+        "coefficients": coefs,
+        "warnings": out.warning_list(),
+        "sidecar": _sidecar(out, spec, regularization, alpha_mode, alpha, alpha_ratio, path_n_alphas, path_eps,
+                            slack, max_iter, tol, bool(non_negative), lambda_n, lambda_l, bvec_frame, K,
+                            radial_order, zeta, dist),
+        "alpha_distribution": dist,
+    }
+    if diag:
+        # src/fit.rs:810-822 — rmse = residual_l2 / sqrt(n_grads)
+        result["rmse"] = (res / np.float32(np.sqrt(np.float32(g.n_grads)))).astype(np.float32)
+        result["r2"] = r2
+        result["residual_l2"] = res
+        result["iterations"] = its
+        result["regularization_kind"] = kind
+        if l1:
+            result["alpha"] = amap
+            result["bic"] = bic
+    return result

@@ -1,0 +1,176 @@
+/*
+ * shorefit — MI355X (gfx950) 3D-SHORE fit behind a C ABI.
+ *
+ * Drop-in for the volume-level SHORE fit of PennLINC/cs_dmri 0.3.1:
+ *   fit_shore(&DwiData, &ShoreFitSpec, on_voxel) -> Result<ShoreFitOutput>
+ *       src/fit.rs:687-805 (called by src/bin/cs-fit.rs:287 and the pyo3
+ *       binding python/src/models.rs:160)
+ *   _cs_dmri.shore_fit(data, bvals, bvecs, mask, **kw) -> dict
+ *       python/src/models.rs:70-198
+ * plus the per-problem solver seams the reference's tests drive:
+ *   FistaSolver::new / fit           src/solver/fista.rs:38-150
+ *   TikhonovSolver::new / fit        src/solver/tikhonov.rs:26-76
+ *   ShoreBasis::design_matrix        src/basis/shore.rs:122-178
+ *
+ * Conventions: plain pointers and sizes only. Volumes are float32 (X,Y,Z,N)
+ * C-order, masks uint8 (X,Y,Z); coefficients come back float32 (X,Y,Z,K)
+ * with zeros outside the mask (src/fit.rs:89,139-142). Outputs are caller
+ * allocated (sizes from shorefit_n_coeffs). Every entry point returns a
+ * SHOREFIT_* status; the message of the last failure on the calling thread
+ * is shorefit_last_error(). Calls are re-entrant: each call owns its device
+ * buffers and runs on the stream it is given (NULL = the default stream).
+ */
+#ifndef SHOREFIT_H
+#define SHOREFIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference's Result<_, CsDmriError> (src/lib.rs:63-83)
+ * maps to SHOREFIT_E_INVALID / SHOREFIT_E_FIT; its panics (non-PD Cholesky,
+ * src/solver/tikhonov.rs:44) map to SHOREFIT_E_NOT_PD. */
+#define SHOREFIT_OK 0
+#define SHOREFIT_E_INVALID 1     /* bad argument (CsDmriError::Other / Dimension) */
+#define SHOREFIT_E_FIT 2         /* fit refused (CsDmriError::Fit: no DWI, single shell) */
+#define SHOREFIT_E_NOT_PD 3      /* Tikhonov normal matrix not positive definite */
+#define SHOREFIT_E_UNSUPPORTED 4 /* option this library does not implement */
+#define SHOREFIT_E_DEVICE 5      /* HIP runtime failure */
+
+/* ShoreRegularization (src/fit.rs:597-612) */
+#define SHOREFIT_REG_L1 1
+#define SHOREFIT_REG_L2 2
+#define SHOREFIT_REG_NONNEG 3 /* AmpNonNeg (ICLS): not implemented -> SHOREFIT_E_UNSUPPORTED */
+
+/* AlphaMode (src/fit.rs:545-552) */
+#define SHOREFIT_ALPHA_FIXED 0
+#define SHOREFIT_ALPHA_RATIO 1
+#define SHOREFIT_ALPHA_PATH_BIC 2
+#define SHOREFIT_ALPHA_L2_ANCHORED 3
+
+/* Arithmetic of the FISTA contraction (device knob; no reference counterpart). */
+#define SHOREFIT_PREC_FP32 0 /* FP32 MFMA every iteration */
+#define SHOREFIT_PREC_FP16 1 /* FP16 MFMA increments + FP32 refresh every refresh_every iterations */
+#define SHOREFIT_PREC_BF16 2 /* BF16 MFMA increments + FP32 refresh */
+
+/* Mirrors ShoreFitSpec + build_alpha_strategy's inputs (src/fit.rs:555-669)
+ * and the keyword set of _cs_dmri.shore_fit (python/src/models.rs:71-75). */
+typedef struct ShorefitSpec {
+    uint32_t radial_order;   /* even, > 0 (python/src/models.rs:145-147) */
+    double zeta;             /* 700 */
+    int32_t regularization;  /* SHOREFIT_REG_* */
+    int32_t alpha_mode;      /* SHOREFIT_ALPHA_* */
+    double alpha;            /* fixed alpha, also the seed alpha (max(alpha,1e-12)) */
+    double alpha_ratio;      /* in (0,1) */
+    uint32_t path_n_alphas;  /* >= 2 */
+    double path_eps;         /* in (0,1); <= 0 selects the mode default (1e-3 / 1e-4) */
+    double slack;            /* >= 0 */
+    uint32_t max_iter;       /* FISTA cap */
+    double tol;              /* relative-change tolerance */
+    int32_t non_negative;
+    double lambda_n;         /* Tikhonov radial weight */
+    double lambda_l;         /* Tikhonov angular weight */
+    int32_t compute_diagnostics;
+    int32_t allow_single_shell;
+    /* device knobs */
+    int32_t precision;       /* SHOREFIT_PREC_* */
+    int32_t refresh_every;   /* FP32 refresh period of the mixed-precision kernel (>= 1) */
+    int32_t reserved[6];
+} ShorefitSpec;
+
+/* Gradient table (src/qspace.rs:74-120). NaN deltas mean "absent": both are
+ * then estimated with TORTOISE's heuristic at Gmax = 80 mT/m (:213-227). */
+typedef struct ShorefitGradients {
+    const double* bvals;  /* N, s/mm^2 */
+    const double* bvecs;  /* N x 3 row-major */
+    int64_t n_grads;
+    double big_delta;     /* seconds, NaN = absent */
+    double small_delta;   /* seconds, NaN = absent */
+    double b0_threshold;  /* 50 */
+} ShorefitGradients;
+
+/* DwiData's volume part (src/io/dwi.rs:22-40). */
+typedef struct ShorefitVolume {
+    const float* data;    /* (X,Y,Z,N) float32, C-order */
+    const uint8_t* mask;  /* (X,Y,Z) uint8, NULL = every voxel */
+    int64_t nx, ny, nz;
+    int32_t on_device;    /* 1: data, mask and every output pointer are device pointers */
+    int32_t reserved;
+} ShorefitVolume;
+
+/* FitResult + VolumeDiagnostics + the sidecar facts of ShoreFitOutput
+ * (src/fit.rs:27-48, :672-683; python/src/models.rs:162-197). */
+typedef struct ShorefitOutput {
+    float* coefficients;          /* (X,Y,Z,K) float32, required */
+    float* r2;                    /* (X,Y,Z) optional (NULL skips) */
+    float* residual_l2;           /* (X,Y,Z) optional */
+    uint32_t* iterations;         /* (X,Y,Z) optional */
+    uint8_t* regularization_kind; /* (X,Y,Z) optional: 1 = L1, 2 = L2 */
+    float* alpha;                 /* (X,Y,Z) optional, L1 only */
+    float* bic;                   /* (X,Y,Z) optional, L1 only */
+    float* rss_l2;                /* (X,Y,Z) optional, L2-anchored only */
+    /* filled in by the library */
+    int32_t n_coeffs;
+    int32_t has_alpha_distribution;
+    double alpha_distribution[3]; /* (median, p10, p90) for per-voxel alpha strategies */
+    double big_delta, small_delta, tau;
+    int32_t delta_source;         /* 0 user supplied, 1 TORTOISE estimate */
+    int32_t n_shells;
+    double dwi_shells[32];        /* mean b of the non-zero shells, ascending */
+    int32_t n_warnings;
+    char warnings[2048];          /* n_warnings messages, each NUL-terminated, back to back */
+    int64_t n_voxels;             /* masked voxels fitted */
+    double lipschitz;             /* sigma_max(A)^2 / N used by FISTA */
+    double fit_ms;                /* device time of the fit kernels */
+} ShorefitOutput;
+
+/* Fill *spec with the cs-fit / ShoreModel defaults (src/fit.rs:647-669):
+ * order 6, zeta 700, L1 + L2-anchored path (20 alphas, eps 1e-4, slack 0.05),
+ * max_iter 1000, tol 1e-6, lambda_n = lambda_l = 1e-8, FP32. */
+void shorefit_spec_default(ShorefitSpec* spec);
+
+/* Whole-volume fit: replaces fit_shore (src/fit.rs:687-805). stream is a
+ * hipStream_t (NULL = default stream). */
+int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitSpec* spec,
+                 ShorefitOutput* out, void* stream);
+
+/* Number of SHORE coefficients at radial_order (ShoreBasis::n_coeffs,
+ * src/basis/shore.rs:29-44, :118-120): 72 / 145 / 256 at order 6 / 8 / 10.
+ * Returns -1 for an odd order. */
+int shorefit_n_coeffs(uint32_t radial_order);
+
+/* ShoreBasis::design_matrix (src/basis/shore.rs:122-178), evaluated on the
+ * device in FP64. A_out: host, N x K row-major float64. */
+int shorefit_design_matrix(const ShorefitGradients* grads, uint32_t radial_order, double zeta, double* A_out);
+
+/* FistaSolver fits of V signals against one caller design (the Solver seam,
+ * src/solver/fista.rs:38-150). Host pointers: A (N x K row-major f64),
+ * signals (V x N f32). Outputs: coef (V x K f32), iterations (V), converged
+ * (V), residual_l2 (V f32); the last three may be NULL. lipschitz_out (may
+ * be NULL) receives L = max(sigma_max^2/N, eps). */
+int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const float* signals, int64_t n_signals,
+                         double alpha, uint32_t max_iter, double tol, int32_t non_negative, int32_t precision,
+                         int32_t refresh_every, float* coef, uint32_t* iterations, uint8_t* converged,
+                         float* residual_l2, double* lipschitz_out);
+
+/* TikhonovSolver fits of V signals (src/solver/tikhonov.rs:26-76). d_n and
+ * d_l are the K regularisation diagonals (either may be NULL = zeros).
+ * Returns SHOREFIT_E_NOT_PD when A^T A + lambda_n D_n + lambda_l D_l is not
+ * positive definite. */
+int shorefit_tikhonov_batch(const double* A, int32_t n_rows, int32_t n_cols, const double* d_n, const double* d_l,
+                            double lambda_n, double lambda_l, const float* signals, int64_t n_signals, float* coef,
+                            float* residual_l2);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* shorefit_last_error(void);
+
+/* Library version string. */
+const char* shorefit_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHOREFIT_H */

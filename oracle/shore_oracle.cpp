@@ -1,0 +1,804 @@
+// SHORE-fit oracle: an FP64 CPU restatement of cs_dmri's per-voxel 3D-SHORE
+// fit path (PennLINC/cs_dmri 0.3.1).
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
+// calls this file; only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg use it, and only as the checker / CPU baseline.
+//
+// Every function names the reference statement it restates (file:line in the
+// reference repository). Arithmetic is FP64 throughout, inputs f32 volumes,
+// outputs f32 maps, exactly as src/fit.rs does.
+//
+// Parity pins (see tests/test_oracle_*.py):
+//   math KATs ............ src/math.rs:168-247 + scipy.special closed forms
+//   design matrix ........ src/basis/shore.rs:205-273 KATs + scipy golden
+//   FISTA ................ 8 sklearn fixtures (tests/data/lasso_fixtures),
+//                          KKT, limits (tests/optimization_{kkt,limits}.rs)
+//   Lipschitz ............ numpy.linalg.svd
+//   Tikhonov ............. normal-equation residual + scipy cho_solve
+//   alpha machinery ...... closed forms (alpha_max zero solution, log path
+//                          endpoints, BIC formula) and brute-force argmin
+//   volume driver ........ per-voxel equality with the single-voxel entry
+//                          points, R^2 closed form, scatter layout
+//   graddev table ........ src/graddev.rs:218-253 KATs
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------- math ----
+
+// src/math.rs:8-37 — exact table up to 20!, gamma beyond.
+double gamma_fn(double x);
+double factorial(unsigned n) {
+    static const double table[21] = {1.0,
+                                     1.0,
+                                     2.0,
+                                     6.0,
+                                     24.0,
+                                     120.0,
+                                     720.0,
+                                     5040.0,
+                                     40320.0,
+                                     362880.0,
+                                     3628800.0,
+                                     39916800.0,
+                                     479001600.0,
+                                     6227020800.0,
+                                     87178291200.0,
+                                     1307674368000.0,
+                                     20922789888000.0,
+                                     355687428096000.0,
+                                     6402373705728000.0,
+                                     121645100408832000.0,
+                                     2432902008176640000.0};
+    if (n < 21) return table[n];
+    return gamma_fn(double(n) + 1.0);
+}
+
+// src/math.rs:40-64 — Lanczos approximation, g = 7, nine terms (the published
+// Godfrey coefficient set), with the reflection formula below 1/2.
+double gamma_fn(double x) {
+    if (x < 0.5) return kPi / (std::sin(kPi * x) * gamma_fn(1.0 - x));
+    static const double c[9] = {0.99999999999980993,  676.5203681218851,     -1259.1392167224028,
+                                771.32342877765313,   -176.61502916214059,   12.507343278686905,
+                                -0.13857109526572012, 9.9843695780195716e-6, 1.5056327351493116e-7};
+    const double g = 7.0;
+    x -= 1.0;
+    double a = c[0];
+    const double t = x + g + 0.5;
+    for (int i = 1; i < 9; ++i) a += c[i] / (x + double(i));
+    return std::sqrt(2.0 * kPi) * std::pow(t, x + 0.5) * std::exp(-t) * a;
+}
+
+// src/math.rs:70-101 — P_l^m(x) with the Condon-Shortley phase, upward
+// recurrence in l from P_m^m.
+double assoc_legendre(unsigned l, unsigned m, double x) {
+    double pmm = 1.0;
+    if (m > 0) {
+        const double somx2 = std::sqrt(std::max((1.0 - x) * (1.0 + x), 0.0));
+        double fact = 1.0;
+        for (unsigned i = 0; i < m; ++i) {
+            pmm *= -fact * somx2;
+            fact += 2.0;
+        }
+    }
+    if (l == m) return pmm;
+    double pmmp1 = x * (2.0 * double(m) + 1.0) * pmm;
+    if (l == m + 1) return pmmp1;
+    double pll = 0.0;
+    for (unsigned ll = m + 2; ll <= l; ++ll) {
+        const double lf = double(ll), mf = double(m);
+        pll = ((2.0 * lf - 1.0) * x * pmmp1 - (lf + mf - 1.0) * pmm) / (lf - mf);
+        pmm = pmmp1;
+        pmmp1 = pll;
+    }
+    return pll;
+}
+
+// src/math.rs:108-125 — L_n^alpha(x) by the three-term recurrence.
+double gen_laguerre(unsigned n, double alpha, double x) {
+    if (n == 0) return 1.0;
+    double lkm1 = 1.0;
+    double lk = 1.0 + alpha - x;
+    if (n == 1) return lk;
+    for (unsigned k = 1; k < n; ++k) {
+        const double kf = double(k);
+        const double lkp1 = ((2.0 * kf + 1.0 + alpha - x) * lk - (kf + alpha) * lkm1) / (kf + 1.0);
+        lkm1 = lk;
+        lk = lkp1;
+    }
+    return lk;
+}
+
+// src/math.rs:157-166 — (r, theta = azimuth, phi = polar).
+void cart2sphere(double x, double y, double z, double* r, double* theta, double* phi) {
+    *r = std::sqrt(x * x + y * y + z * z);
+    *theta = std::atan2(y, x);
+    *phi = (*r > 0.0) ? std::acos(std::min(1.0, std::max(-1.0, z / *r))) : 0.0;
+}
+
+// -------------------------------------------------------------- basis ----
+
+struct Index3 {
+    unsigned n, ell;
+    int m;
+};
+
+// src/basis/shore.rs:29-44 — for n in 0..=R, even ell <= n, m in -ell..=ell.
+std::vector<Index3> shore_indices(unsigned radial_order) {
+    std::vector<Index3> idx;
+    for (unsigned n = 0; n <= radial_order; ++n)
+        for (unsigned ell = 0; ell <= n; ell += 2)
+            for (int m = -int(ell); m <= int(ell); ++m) idx.push_back({n, ell, m});
+    return idx;
+}
+
+// src/basis/shore.rs:56-61
+double kappa(double zeta, unsigned n, unsigned ell) {
+    const double num = 2.0 * factorial(n - ell);
+    const double den = std::pow(zeta, 1.5) * gamma_fn(double(n) + 1.5);
+    return std::sqrt(num / den);
+}
+
+// src/basis/shore.rs:64-70
+double shore_radial(unsigned n, unsigned ell, double q, double zeta) {
+    const double x = q * q / zeta;
+    const double lag = gen_laguerre(n - ell, double(ell) + 0.5, x);
+    const double env = std::exp(-x * 0.5);
+    const double pw = std::pow(x, double(ell) / 2.0);
+    return kappa(zeta, n, ell) * lag * env * pw;
+}
+
+// src/basis/shore.rs:76-115 — BrainSuite ordering: cos terms k = ell..1,
+// then P_ell^0, then signed sin terms k = 1..ell.
+void brainsuite_sh_block(unsigned ell, double theta, double phi, double* out) {
+    if (ell == 0) {
+        out[0] = 1.0 / std::sqrt(4.0 * kPi);
+        return;
+    }
+    const double cphi = std::cos(phi);
+    const double tl1 = 2.0 * double(ell) + 1.0;
+    std::vector<double> p(ell + 1);
+    for (unsigned k = 0; k <= ell; ++k) {
+        const double norm = std::sqrt(tl1 / (4.0 * kPi) * factorial(ell - k) / factorial(ell + k));
+        p[k] = norm * assoc_legendre(ell, k, cphi);
+    }
+    const double sqrt2 = std::sqrt(2.0);
+    for (unsigned k = 1; k <= ell; ++k) out[ell - k] = sqrt2 * p[k] * std::cos(double(k) * theta);
+    out[ell] = p[0];
+    for (unsigned k = 1; k <= ell; ++k) {
+        const double sign = ((k + 1) % 2 == 0) ? 1.0 : -1.0;
+        out[ell + k] = sqrt2 * sign * p[k] * std::sin(double(k) * theta);
+    }
+}
+
+// src/qspace.rs:130-149 — tau = Delta - delta/3; q = sqrt(b / (4 pi^2 tau)),
+// zero at or below the b0 threshold.
+void qvals(const double* bvals, int n, double big_delta, double small_delta, double b0_thr, double* q) {
+    const double tau = big_delta - small_delta / 3.0;
+    const double denom = 4.0 * kPi * kPi * tau;
+    for (int i = 0; i < n; ++i) q[i] = (bvals[i] <= b0_thr) ? 0.0 : std::sqrt(bvals[i] / denom);
+}
+
+// src/basis/shore.rs:122-178 — row-major N x K output.
+void design_matrix(unsigned radial_order, double zeta, const double* q, const double* bvecs, int n_rows,
+                   double* A) {
+    const auto idx = shore_indices(radial_order);
+    const int K = int(idx.size());
+    std::vector<double> r(n_rows), th(n_rows), ph(n_rows);
+    for (int i = 0; i < n_rows; ++i) {
+        double rr, t, p;
+        cart2sphere(bvecs[3 * i] * q[i], bvecs[3 * i + 1] * q[i], bvecs[3 * i + 2] * q[i], &rr, &t, &p);
+        if (std::isnan(t)) t = 0.0;
+        r[i] = rr;
+        th[i] = t;
+        ph[i] = p;
+    }
+    std::vector<double> block(2 * radial_order + 1);
+    int col = 0;
+    for (unsigned n = 0; n <= radial_order; ++n) {
+        for (unsigned ell = 0; ell <= n; ell += 2) {
+            const int bs = int(2 * ell + 1);
+            for (int i = 0; i < n_rows; ++i) {
+                const double rad = shore_radial(n, ell, r[i], zeta);
+                brainsuite_sh_block(ell, th[i], ph[i], block.data());
+                for (int k = 0; k < bs; ++k) A[size_t(i) * K + col + k] = rad * block[k];
+            }
+            col += bs;
+        }
+    }
+}
+
+// ------------------------------------------------------------- linalg ----
+
+// Largest eigenvalue of a symmetric K x K matrix by cyclic Jacobi rotations
+// (textbook algorithm, Golub & Van Loan 8.5). Used for sigma_max(A)^2, the
+// quantity the reference reads off nalgebra's SVD (src/solver/fista.rs:45-51).
+double sym_max_eig(std::vector<double> a, int K) {
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int p = 0; p < K; ++p) {
+            diag += a[size_t(p) * K + p] * a[size_t(p) * K + p];
+            for (int q = p + 1; q < K; ++q) off += a[size_t(p) * K + q] * a[size_t(p) * K + q];
+        }
+        if (off <= 1e-32 * diag || off == 0.0) break;
+        for (int p = 0; p < K; ++p) {
+            for (int q = p + 1; q < K; ++q) {
+                const double apq = a[size_t(p) * K + q];
+                if (apq == 0.0) continue;
+                const double app = a[size_t(p) * K + p], aqq = a[size_t(q) * K + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < K; ++k) {
+                    const double akp = a[size_t(k) * K + p], akq = a[size_t(k) * K + q];
+                    a[size_t(k) * K + p] = c * akp - s * akq;
+                    a[size_t(k) * K + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < K; ++k) {
+                    const double apk = a[size_t(p) * K + k], aqk = a[size_t(q) * K + k];
+                    a[size_t(p) * K + k] = c * apk - s * aqk;
+                    a[size_t(q) * K + k] = s * apk + c * aqk;
+                }
+            }
+        }
+    }
+    double mx = -std::numeric_limits<double>::infinity();
+    for (int p = 0; p < K; ++p) mx = std::max(mx, a[size_t(p) * K + p]);
+    return mx;
+}
+
+// G = A^T A for row-major A (N x K).
+std::vector<double> gram(const double* A, int N, int K) {
+    std::vector<double> g(size_t(K) * K, 0.0);
+    for (int i = 0; i < N; ++i)
+        for (int p = 0; p < K; ++p) {
+            const double ap = A[size_t(i) * K + p];
+            for (int q = 0; q < K; ++q) g[size_t(p) * K + q] += ap * A[size_t(i) * K + q];
+        }
+    return g;
+}
+
+// src/solver/fista.rs:38-62 — L = max(sigma_max^2 / n, f64::EPSILON).
+double lipschitz(const double* A, int N, int K) {
+    const double smax2 = std::max(0.0, sym_max_eig(gram(A, N, K), K));
+    return std::max(smax2 / double(N), std::numeric_limits<double>::epsilon());
+}
+
+double nrm2(const double* v, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += v[i] * v[i];
+    return std::sqrt(s);
+}
+
+struct Diag {
+    uint32_t iterations;
+    double residual_l2;
+    bool converged;
+    double alpha;
+    uint8_t kind;
+};
+
+// src/solver/fista.rs:70-143 — statement-for-statement FISTA with an optional
+// warm start (used only when its length matches, :76-79).
+Diag fista(const double* A, int N, int K, const double* s, double alpha, uint32_t max_iter, double tol,
+           bool non_negative, double L, const double* warm, double* x_out) {
+    std::vector<double> x(K, 0.0), y(K), my(N), g(K), xn(K), yn(K);
+    if (warm) std::copy(warm, warm + K, x.begin());
+    y = x;
+    double t = 1.0;
+    const double step = 1.0 / L;
+    const double thr = step * alpha;
+    const double ninv = 1.0 / double(N);
+    uint32_t iters = 0;
+    bool converged = false;
+    for (uint32_t it = 0; it < max_iter; ++it) {
+        for (int i = 0; i < N; ++i) {  // m_y = M y ; resid = m_y - s
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) acc += A[size_t(i) * K + k] * y[k];
+            my[i] = acc - s[i];
+        }
+        for (int k = 0; k < K; ++k) {  // grad = M^T resid * (1/n)
+            double acc = 0.0;
+            for (int i = 0; i < N; ++i) acc += A[size_t(i) * K + k] * my[i];
+            g[k] = acc * ninv;
+        }
+        for (int k = 0; k < K; ++k) {  // soft threshold (strict inequalities)
+            const double v = y[k] - step * g[k];
+            double sv = (v > thr) ? v - thr : (v < -thr) ? v + thr : 0.0;
+            xn[k] = non_negative ? std::max(sv, 0.0) : sv;
+        }
+        const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+        const double mom = (t - 1.0) / tn;
+        double dx2 = 0.0;
+        for (int k = 0; k < K; ++k) {
+            yn[k] = xn[k] + mom * (xn[k] - x[k]);
+            dx2 += (xn[k] - x[k]) * (xn[k] - x[k]);
+        }
+        const double dx = std::sqrt(dx2);
+        const double den = std::max(nrm2(xn.data(), K), 1e-12);
+        const double rel = dx / den;
+        x.swap(xn);
+        y.swap(yn);
+        t = tn;
+        iters = it + 1;
+        if (rel < tol) {
+            converged = true;
+            break;
+        }
+    }
+    double r2 = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < K; ++k) acc += A[size_t(i) * K + k] * x[k];
+        r2 += (s[i] - acc) * (s[i] - acc);
+    }
+    std::copy(x.begin(), x.end(), x_out);
+    return Diag{iters, std::sqrt(r2), converged, alpha, 1};
+}
+
+// src/solver/tikhonov.rs:26-53 — lower Cholesky factor of
+// A^T A + lambda_n D_n + lambda_l D_l; returns false when not PD (the
+// reference panics at :44).
+bool tikhonov_factor(const double* A, int N, int K, const double* dn, const double* dl, double lam_n, double lam_l,
+                     std::vector<double>& Lf) {
+    Lf = gram(A, N, K);
+    for (int i = 0; i < K; ++i) {
+        if (dn) Lf[size_t(i) * K + i] += lam_n * dn[i];
+        if (dl) Lf[size_t(i) * K + i] += lam_l * dl[i];
+    }
+    for (int j = 0; j < K; ++j) {
+        double d = Lf[size_t(j) * K + j];
+        for (int k = 0; k < j; ++k) d -= Lf[size_t(j) * K + k] * Lf[size_t(j) * K + k];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        Lf[size_t(j) * K + j] = d;
+        for (int i = j + 1; i < K; ++i) {
+            double v = Lf[size_t(i) * K + j];
+            for (int k = 0; k < j; ++k) v -= Lf[size_t(i) * K + k] * Lf[size_t(j) * K + k];
+            Lf[size_t(i) * K + j] = v / d;
+        }
+    }
+    for (int i = 0; i < K; ++i)
+        for (int j = i + 1; j < K; ++j) Lf[size_t(i) * K + j] = 0.0;
+    return true;
+}
+
+// src/solver/tikhonov.rs:59-76 — c = chol.solve(M^T s); residual ||s - M c||.
+Diag tikhonov_fit(const double* A, int N, int K, const std::vector<double>& Lf, const double* s, double* c) {
+    std::vector<double> z(K);
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (int i = 0; i < N; ++i) acc += A[size_t(i) * K + k] * s[i];
+        z[k] = acc;
+    }
+    for (int i = 0; i < K; ++i) {  // L w = rhs
+        double v = z[i];
+        for (int k = 0; k < i; ++k) v -= Lf[size_t(i) * K + k] * z[k];
+        z[i] = v / Lf[size_t(i) * K + i];
+    }
+    for (int i = K - 1; i >= 0; --i) {  // L^T c = w
+        double v = z[i];
+        for (int k = i + 1; k < K; ++k) v -= Lf[size_t(k) * K + i] * z[k];
+        z[i] = v / Lf[size_t(i) * K + i];
+    }
+    double r2 = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k < K; ++k) acc += A[size_t(i) * K + k] * z[k];
+        r2 += (s[i] - acc) * (s[i] - acc);
+    }
+    std::copy(z.begin(), z.end(), c);
+    return Diag{1, std::sqrt(r2), true, 0.0, 2};
+}
+
+// ------------------------------------------------------- alpha machinery ----
+
+// src/solver/alpha.rs:47-53
+double alpha_max(const double* A, int N, int K, const double* s) {
+    if (N == 0) return 0.0;
+    double m = 0.0;
+    for (int k = 0; k < K; ++k) {
+        double acc = 0.0;
+        for (int i = 0; i < N; ++i) acc += A[size_t(i) * K + k] * s[i];
+        m = std::max(m, std::fabs(acc));
+    }
+    return m / double(N);
+}
+
+// src/solver/alpha.rs:57-69
+void log_path(double amax, int n, double eps, double* out) {
+    const double lmax = std::log(amax), lmin = std::log(amax * eps);
+    for (int i = 0; i < n; ++i) {
+        const double t = double(i) / double(n - 1);
+        out[i] = std::exp(lmax + t * (lmin - lmax));
+    }
+}
+
+// src/solver/alpha.rs:99-103
+double bic(int n_grads, double rss, int k_nonzero) {
+    const double n = double(n_grads);
+    const double rc = std::max(rss, std::numeric_limits<double>::min());
+    return n * std::log(rc / n) + std::log(n) * double(k_nonzero);
+}
+
+int count_nonzero(const double* c, int K) {
+    int k = 0;
+    for (int i = 0; i < K; ++i) k += (c[i] != 0.0);
+    return k;
+}
+
+// Strategy tags mirror AlphaStrategy (src/solver/alpha.rs:195-204).
+enum { kFixed = 0, kAlphaRatio = 1, kPathBic = 2, kL2Anchored = 3 };
+
+struct VoxelResult {
+    double alpha, residual_l2, bic, rss_l2;
+    uint32_t iterations;
+    bool converged;
+};
+
+struct Problem {
+    const double* A;
+    int N, K;
+    double L;
+    uint32_t max_iter;
+    double tol;
+    bool nonneg;
+    int strategy;
+    double alpha, ratio, path_eps, slack;
+    int path_n;
+    const std::vector<double>* tik;  // Cholesky factor for the L2 reference
+};
+
+// src/solver/alpha.rs:207-318 (resolve / resolve_with_l2_ref), including the
+// all-zero-signal short circuit (:178-189, :231-233) and fit_path's warm
+// starts (src/solver/fista.rs:165-180).
+VoxelResult resolve(const Problem& P, const double* s, double* coef) {
+    const int N = P.N, K = P.K;
+    VoxelResult out{0, 0, 0, 0, 0, true};
+    auto zero = [&]() {
+        std::fill(coef, coef + K, 0.0);
+        return VoxelResult{0.0, 0.0, 0.0, 0.0, 0, true};
+    };
+    if (P.strategy == kFixed || P.strategy == kAlphaRatio) {
+        double a = P.alpha;
+        if (P.strategy == kAlphaRatio) {
+            const double amax = alpha_max(P.A, N, K, s);
+            if (amax == 0.0) return zero();
+            a = amax * P.ratio;
+        }
+        Diag d = fista(P.A, N, K, s, a, P.max_iter, P.tol, P.nonneg, P.L, nullptr, coef);
+        out.alpha = a;
+        out.residual_l2 = d.residual_l2;
+        out.iterations = d.iterations;
+        out.converged = d.converged;
+        out.bic = bic(N, d.residual_l2 * d.residual_l2, count_nonzero(coef, K));
+        return out;
+    }
+    const double amax = alpha_max(P.A, N, K, s);
+    if (amax == 0.0) return zero();
+    std::vector<double> alphas(P.path_n);
+    log_path(amax, P.path_n, P.path_eps, alphas.data());
+    std::vector<double> coefs(size_t(P.path_n) * K);
+    std::vector<Diag> diags(P.path_n);
+    for (int i = 0; i < P.path_n; ++i) {
+        const double* warm = (i == 0) ? nullptr : &coefs[size_t(i - 1) * K];
+        diags[i] = fista(P.A, N, K, s, alphas[i], P.max_iter, P.tol, P.nonneg, P.L, warm, &coefs[size_t(i) * K]);
+    }
+    int idx = 0;
+    if (P.strategy == kPathBic) {  // src/solver/alpha.rs:107-128
+        double best = std::numeric_limits<double>::infinity();
+        for (int i = 0; i < P.path_n; ++i) {
+            const double rss = diags[i].residual_l2 * diags[i].residual_l2;
+            const double b = bic(N, rss, count_nonzero(&coefs[size_t(i) * K], K));
+            if (b < best) {
+                best = b;
+                idx = i;
+            }
+        }
+    } else {  // src/solver/alpha.rs:140-165
+        std::vector<double> cl2(K);
+        const Diag d2 = tikhonov_fit(P.A, N, K, *P.tik, s, cl2.data());
+        const double rss_l2 = d2.residual_l2 * d2.residual_l2;
+        out.rss_l2 = rss_l2;
+        const double thr = (1.0 + P.slack) * rss_l2;
+        idx = -1;
+        for (int i = 0; i < P.path_n; ++i) {
+            if (diags[i].residual_l2 * diags[i].residual_l2 <= thr) {
+                idx = i;
+                break;
+            }
+        }
+        if (idx < 0) {
+            idx = 0;
+            double best = std::numeric_limits<double>::infinity();
+            for (int i = 0; i < P.path_n; ++i) {
+                const double rss = diags[i].residual_l2 * diags[i].residual_l2;
+                if (rss < best) {
+                    best = rss;
+                    idx = i;
+                }
+            }
+        }
+    }
+    std::copy(&coefs[size_t(idx) * K], &coefs[size_t(idx) * K] + K, coef);
+    const Diag& d = diags[idx];
+    out.alpha = alphas[idx];
+    out.residual_l2 = d.residual_l2;
+    out.iterations = d.iterations;
+    out.converged = d.converged;
+    out.bic = bic(N, d.residual_l2 * d.residual_l2, count_nonzero(coef, K));
+    return out;
+}
+
+}  // namespace
+
+// ======================================================== C entry points ====
+extern "C" {
+
+int orc_n_coeffs(unsigned radial_order) { return int(shore_indices(radial_order).size()); }
+
+void orc_shore_indices(unsigned radial_order, int32_t* out_nlm) {
+    const auto idx = shore_indices(radial_order);
+    for (size_t i = 0; i < idx.size(); ++i) {
+        out_nlm[3 * i] = int32_t(idx[i].n);
+        out_nlm[3 * i + 1] = int32_t(idx[i].ell);
+        out_nlm[3 * i + 2] = idx[i].m;
+    }
+}
+
+double orc_factorial(unsigned n) { return factorial(n); }
+double orc_gamma(double x) { return gamma_fn(x); }
+double orc_assoc_legendre(unsigned l, unsigned m, double x) { return assoc_legendre(l, m, x); }
+double orc_gen_laguerre(unsigned n, double a, double x) { return gen_laguerre(n, a, x); }
+double orc_kappa(double zeta, unsigned n, unsigned ell) { return kappa(zeta, n, ell); }
+void orc_brainsuite_sh_block(unsigned ell, double theta, double phi, double* out) {
+    brainsuite_sh_block(ell, theta, phi, out);
+}
+
+// src/qspace.rs:213-227 — TORTOISE (Delta, delta) heuristic.
+void orc_estimate_deltas_tortoise(const double* bvals, int n, double gmax, double* big, double* small) {
+    double mb = 0.0;
+    for (int i = 0; i < n; ++i) mb = std::max(mb, bvals[i]);
+    if (mb <= 0.0) {
+        const double sm = 1.0;
+        *big = 3.0 * sm * 1e-3;
+        *small = sm * 1e-3;
+        return;
+    }
+    const double gyro = 267.51532e6;
+    const double temp = mb / gyro / gyro / gmax / gmax / 2.0 * 1e6;
+    const double small_ms = std::pow(temp, 1.0 / 3.0) * 1000.0;
+    *big = small_ms * 3.0 * 1e-3;
+    *small = small_ms * 1e-3;
+}
+
+void orc_qvals(const double* bvals, int n, double big_delta, double small_delta, double b0_thr, double* q) {
+    qvals(bvals, n, big_delta, small_delta, b0_thr, q);
+}
+
+// src/qspace.rs:171-197 + src/fit.rs:615-617 — mean b-values of the non-zero
+// shells, ascending. Returns the shell count; writes at most max_out values.
+int orc_dwi_shells(const double* bvals, int n, double b0_thr, double tol, double* out, int max_out) {
+    std::vector<double> b;
+    std::vector<int> cnt;
+    for (int i = 0; i < n; ++i) {
+        if (bvals[i] <= b0_thr) continue;
+        bool placed = false;
+        for (size_t s = 0; s < b.size(); ++s) {
+            if (std::fabs(b[s] - bvals[i]) <= tol) {
+                const double c = double(cnt[s]);
+                b[s] = (b[s] * c + bvals[i]) / (c + 1.0);
+                cnt[s] += 1;
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) {
+            b.push_back(bvals[i]);
+            cnt.push_back(1);
+        }
+    }
+    std::stable_sort(b.begin(), b.end());
+    for (int i = 0; i < int(b.size()) && i < max_out; ++i) out[i] = b[i];
+    return int(b.size());
+}
+
+// Design matrix from raw (bvals, bvecs, deltas): row-major N x K.
+void orc_design_matrix(unsigned radial_order, double zeta, const double* bvals, const double* bvecs, int n,
+                       double big_delta, double small_delta, double b0_thr, double* A) {
+    std::vector<double> q(n);
+    qvals(bvals, n, big_delta, small_delta, b0_thr, q.data());
+    design_matrix(radial_order, zeta, q.data(), bvecs, n, A);
+}
+
+// src/basis/shore.rs:180-195
+void orc_regularization(unsigned radial_order, double* dn, double* dl) {
+    const auto idx = shore_indices(radial_order);
+    for (size_t i = 0; i < idx.size(); ++i) {
+        const double nn = double(idx[i].n * (idx[i].n + 1));
+        const double ll = double(idx[i].ell * (idx[i].ell + 1));
+        dn[i] = nn * nn;
+        dl[i] = ll * ll;
+    }
+}
+
+double orc_lipschitz(const double* A, int N, int K) { return lipschitz(A, N, K); }
+double orc_sym_max_eig(const double* S, int K) { return sym_max_eig(std::vector<double>(S, S + size_t(K) * K), K); }
+
+// One FISTA fit (src/solver/fista.rs:70-143). L <= 0 computes the Lipschitz
+// constant as FistaSolver::new does. diag_out = {iterations, residual_l2,
+// converged, alpha}.
+void orc_fista(const double* A, int N, int K, const double* s, double alpha, uint32_t max_iter, double tol,
+               int non_negative, double L, const double* warm, double* coef, double* diag_out) {
+    if (!(L > 0.0)) L = lipschitz(A, N, K);
+    Diag d = fista(A, N, K, s, alpha, max_iter, tol, non_negative != 0, L, warm, coef);
+    diag_out[0] = d.iterations;
+    diag_out[1] = d.residual_l2;
+    diag_out[2] = d.converged ? 1.0 : 0.0;
+    diag_out[3] = d.alpha;
+}
+
+// Tikhonov fit of V signals (row-major V x N) -> coef V x K, residual_l2 V.
+// Returns 0, or -1 when the normal matrix is not positive definite.
+int orc_tikhonov(const double* A, int N, int K, const double* dn, const double* dl, double lam_n, double lam_l,
+                 const double* S, int V, double* C, double* resid) {
+    std::vector<double> Lf;
+    if (!tikhonov_factor(A, N, K, dn, dl, lam_n, lam_l, Lf)) return -1;
+    for (int v = 0; v < V; ++v) {
+        Diag d = tikhonov_fit(A, N, K, Lf, S + size_t(v) * N, C + size_t(v) * K);
+        if (resid) resid[v] = d.residual_l2;
+    }
+    return 0;
+}
+
+double orc_alpha_max(const double* A, int N, int K, const double* s) { return alpha_max(A, N, K, s); }
+void orc_log_path(double amax, int n, double eps, double* out) { log_path(amax, n, eps, out); }
+double orc_bic(int n_grads, double rss, int k) { return bic(n_grads, rss, k); }
+
+// Whole-volume fit mirroring src/fit.rs:74-175 (L2), :219-362 (L1 alpha
+// strategy) and :372-527 (L2-anchored), with voxels collected in x->y->z
+// order (src/voxel_loop.rs:28-42) and a per-task thread pool in place of
+// rayon. A is row-major N x K (built by the caller).
+//
+// reg: 1 = L1, 2 = L2. strategy: 0 fixed, 1 alpha-ratio, 2 path-bic,
+// 3 l2-anchored. Output maps may be NULL. Returns 0, -1 if the Tikhonov
+// normal matrix is not PD.
+int orc_fit_volume(const float* data, const uint8_t* mask, int nx, int ny, int nz, int N, const double* A, int K,
+                   int reg, int strategy, double alpha, double alpha_ratio, int path_n, double path_eps,
+                   double slack, uint32_t max_iter, double tol, int non_negative, const double* dn,
+                   const double* dl, double lam_n, double lam_l, int n_threads, float* coef_out, float* r2_out,
+                   float* resid_out, uint32_t* iters_out, uint8_t* kind_out, float* alpha_out, float* bic_out,
+                   float* rss_l2_out, double* alpha_dist_out) {
+    std::vector<int64_t> work;
+    for (int64_t x = 0; x < nx; ++x)
+        for (int64_t y = 0; y < ny; ++y)
+            for (int64_t z = 0; z < nz; ++z) {
+                const int64_t vi = (x * ny + y) * nz + z;
+                if (!mask || mask[vi]) work.push_back(vi);
+            }
+    const int64_t nvox = int64_t(nx) * ny * nz;
+    std::fill(coef_out, coef_out + nvox * K, 0.0f);
+    if (r2_out) std::fill(r2_out, r2_out + nvox, 0.0f);
+    if (resid_out) std::fill(resid_out, resid_out + nvox, 0.0f);
+    if (iters_out) std::fill(iters_out, iters_out + nvox, 0u);
+    if (kind_out) std::fill(kind_out, kind_out + nvox, uint8_t(0));
+    if (alpha_out) std::fill(alpha_out, alpha_out + nvox, 0.0f);
+    if (bic_out) std::fill(bic_out, bic_out + nvox, 0.0f);
+    if (rss_l2_out) std::fill(rss_l2_out, rss_l2_out + nvox, 0.0f);
+
+    std::vector<double> tik;
+    const bool need_tik = (reg == 2) || (reg == 1 && strategy == kL2Anchored);
+    if (need_tik && !tikhonov_factor(A, N, K, dn, dl, lam_n, lam_l, tik)) return -1;
+    Problem P{A, N, K, 0.0, max_iter, tol, non_negative != 0, strategy, alpha, alpha_ratio, path_eps, slack, path_n,
+              &tik};
+    if (reg == 1) P.L = lipschitz(A, N, K);
+
+    std::vector<double> alphas_seen(work.size(), 0.0);
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        std::vector<double> s(N), c(K);
+        for (;;) {
+            const size_t w = next.fetch_add(1);
+            if (w >= work.size()) break;
+            const int64_t vi = work[w];
+            const float* src = data + vi * N;
+            double mean = 0.0;
+            for (int i = 0; i < N; ++i) {
+                s[i] = double(src[i]);
+                mean += s[i];
+            }
+            mean /= double(N);
+            double resid_l2, a = 0.0, b = 0.0, rss_l2 = 0.0;
+            uint32_t its;
+            if (reg == 2) {
+                Diag d = tikhonov_fit(A, N, K, tik, s.data(), c.data());
+                resid_l2 = d.residual_l2;
+                its = d.iterations;
+            } else {
+                if (strategy == kL2Anchored) {
+                    std::vector<double> cl2(K);
+                    Diag d2 = tikhonov_fit(A, N, K, tik, s.data(), cl2.data());
+                    rss_l2 = d2.residual_l2 * d2.residual_l2;
+                }
+                VoxelResult r = resolve(P, s.data(), c.data());
+                resid_l2 = r.residual_l2;
+                its = r.iterations;
+                a = r.alpha;
+                b = r.bic;
+            }
+            double ss_tot = 0.0;
+            for (int i = 0; i < N; ++i) ss_tot += (s[i] - mean) * (s[i] - mean);
+            const double ss_res = resid_l2 * resid_l2;
+            const double r2 = ss_tot > 0.0 ? 1.0 - ss_res / ss_tot : 0.0;
+            for (int k = 0; k < K; ++k) coef_out[vi * K + k] = float(c[k]);
+            if (r2_out) r2_out[vi] = float(r2);
+            if (resid_out) resid_out[vi] = float(resid_l2);
+            if (iters_out) iters_out[vi] = its;
+            if (kind_out) kind_out[vi] = uint8_t(reg == 2 ? 2 : 1);
+            if (reg == 1) {
+                if (alpha_out) alpha_out[vi] = float(a);
+                if (bic_out) bic_out[vi] = float(b);
+                if (rss_l2_out && strategy == kL2Anchored) rss_l2_out[vi] = float(rss_l2);
+                alphas_seen[w] = a;
+            }
+        }
+    };
+    if (n_threads <= 0) n_threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < n_threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+
+    // src/fit.rs:334-352 — alpha percentiles at index round((len-1) q).
+    if (alpha_dist_out && reg == 1 && strategy != kFixed) {
+        std::vector<double> srt = alphas_seen;
+        std::sort(srt.begin(), srt.end());
+        if (srt.empty()) {
+            alpha_dist_out[0] = alpha_dist_out[1] = alpha_dist_out[2] = 0.0;
+        } else {
+            auto pct = [&](double q) {
+                size_t i = size_t(std::llround(double(srt.size() - 1) * q));
+                return srt[std::min(i, srt.size() - 1)];
+            };
+            alpha_dist_out[0] = pct(0.5);
+            alpha_dist_out[1] = pct(0.10);
+            alpha_dist_out[2] = pct(0.90);
+        }
+    }
+    return 0;
+}
+
+// src/graddev.rs:160-185 — effective table for one deviation matrix T
+// (row-major), with the frame rotation R (row-major) and voxel-frame bvecs.
+void orc_effective_table(const double* T, const double* R, const double* bvals, const double* bvecs_vox,
+                         const double* bvecs_nominal, int n, double b0_thr, double* b_out, double* dir_out) {
+    for (int i = 0; i < n; ++i) {
+        const double* g = bvecs_vox + 3 * i;
+        const double gn = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        if (!(bvals[i] > b0_thr) || !(gn > 1e-12)) {
+            b_out[i] = bvals[i];
+            for (int k = 0; k < 3; ++k) dir_out[3 * i + k] = bvecs_nominal[3 * i + k];
+            continue;
+        }
+        double u[3] = {g[0] / gn, g[1] / gn, g[2] / gn};
+        double q[3];
+        for (int r = 0; r < 3; ++r) q[r] = T[0 * 3 + r] * u[0] + T[1 * 3 + r] * u[1] + T[2 * 3 + r] * u[2];
+        double d[3];
+        for (int r = 0; r < 3; ++r) d[r] = R[r * 3 + 0] * q[0] + R[r * 3 + 1] * q[1] + R[r * 3 + 2] * q[2];
+        const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        b_out[i] = bvals[i] * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        for (int k = 0; k < 3; ++k) dir_out[3 * i + k] = d[k] / dn;
+    }
+}
+
+}  // extern "C"
