@@ -27,9 +27,16 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 namespace {
 
-// Padded coefficient count: the smallest instantiated multiple of 16.
-int pad_coeffs(int K) {
+// Padded coefficient count: the smallest instantiated multiple of 16 (of 32
+// for the mixed-precision kernel, whose FP16/BF16 MFMA reduces 32 at a time).
+int pad_coeffs(int K, bool mixed = false) {
     static const int mbs[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+    static const int mbs_mixed[] = {2, 4, 6, 8, 10, 16};
+    if (mixed) {
+        for (int mb : mbs_mixed)
+            if (16 * mb >= K) return 16 * mb;
+        return -1;
+    }
     for (int mb : mbs)
         if (16 * mb >= K) return 16 * mb;
     return -1;
@@ -103,11 +110,11 @@ struct Design {
     double L = 0.0;     // Lipschitz constant sigma_max^2 / N (floored at eps)
 };
 
-void upload_design(Design& d, const double* A_host, int N, int K, hipStream_t s) {
+void upload_design(Design& d, const double* A_host, int N, int K, hipStream_t s, bool mixed = false) {
     d.N = N;
     d.K = K;
     d.Np = pad16(N);
-    d.Kp = pad_coeffs(K);
+    d.Kp = pad_coeffs(K, mixed);
     if (d.Kp < 0)
         throw Error{SHOREFIT_E_UNSUPPORTED, "at most 256 coefficients (radial order 10) are supported, got " +
                                                 std::to_string(K)};
@@ -116,13 +123,13 @@ void upload_design(Design& d, const double* A_host, int N, int K, hipStream_t s)
 }
 
 void build_design(Design& d, const ShorefitGradients& g, double big, double small, uint32_t order, double zeta,
-                  hipStream_t s) {
+                  hipStream_t s, bool mixed = false) {
     const int N = int(g.n_grads);
     const int K = n_coeffs(order);
     d.N = N;
     d.K = K;
     d.Np = pad16(N);
-    d.Kp = pad_coeffs(K);
+    d.Kp = pad_coeffs(K, mixed);
     if (d.Kp < 0)
         throw Error{SHOREFIT_E_UNSUPPORTED,
                     "radial order " + std::to_string(order) + " (" + std::to_string(K) +
@@ -162,12 +169,55 @@ void pack_fista(const Design& d, FistaOps& ops, int precision, hipStream_t s) {
     ops.step = 1.0 / d.L;
     const double c = ops.step / double(d.N);
     ops.M.alloc(size_t(d.Kp) * d.Kp);
-    ops.AT.alloc(size_t(d.Kp) * d.Np);
     ops.A.alloc(size_t(d.Np) * d.Kp);
     launch_pack_M(d.G.p, d.K, d.Kp, c, ops.M.p, s);
-    launch_pack_rowblock(d.A.p, d.K, d.N, d.K, /*transpose=*/true, c, d.Kp, d.Np, ops.AT.p, s);
     launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
-    (void)precision;
+    if (precision != SHOREFIT_PREC_FP32) {
+        ops.Mlow.alloc(size_t(d.Kp) * d.Kp);
+        launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, precision == SHOREFIT_PREC_BF16, ops.Mlow.p, s);
+    }
+}
+
+// Device buffers of one FISTA solve: right-hand sides, and for the
+// mixed-precision persistent kernel its momentum table, work counter and
+// work-order results.
+struct FistaWork {
+    DevBuf<float> B, mean, mom, X, alpha;
+    DevBuf<unsigned int> counter;
+    DevBuf<uint32_t> its;
+    DevBuf<uint8_t> status;
+};
+
+void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& ops, int64_t V, hipStream_t s) {
+    w.B.alloc(size_t(V) * d.Kp);
+    w.mean.alloc(V);
+    L.A64 = d.A.p;
+    L.c = ops.step / double(d.N);
+    L.Bbuf = w.B.p;
+    L.meanbuf = w.mean.p;
+    L.Mlow = ops.Mlow.p;
+    if (L.precision == SHOREFIT_PREC_FP32) return;
+    // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112), in FP64
+    std::vector<float> mom(std::max<uint32_t>(L.max_iter, 1));
+    double t = 1.0;
+    for (size_t k = 0; k < mom.size(); ++k) {
+        const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+        mom[k] = float((t - 1.0) / tn);
+        t = tn;
+    }
+    w.mom.alloc(mom.size());
+    SF_HIP(hipMemcpyAsync(w.mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
+    w.counter.alloc(1);
+    w.X.alloc(size_t(V) * d.Kp);
+    w.its.alloc(V);
+    w.status.alloc(V);
+    w.alpha.alloc(V);
+    L.mom = w.mom.p;
+    L.counter = w.counter.p;
+    L.Xbuf = w.X.p;
+    L.itsbuf = w.its.p;
+    L.statusbuf = w.status.p;
+    L.alphabuf = w.alpha.p;
 }
 
 // P = (A^T A + lambda_n D_n + lambda_l D_l)^-1 A^T in FP64 (Cholesky on the
@@ -303,8 +353,12 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         if (!A || !signals || !coef || n_rows <= 0 || n_cols <= 0 || n_signals < 0)
             throw Error{SHOREFIT_E_INVALID, "invalid fista_batch arguments"};
         hipStream_t s = nullptr;
+        if (precision != SHOREFIT_PREC_FP32 && precision != SHOREFIT_PREC_FP16 && precision != SHOREFIT_PREC_BF16)
+            throw Error{SHOREFIT_E_INVALID, "precision must be fp32, fp16 or bf16"};
+        if (max_iter == 0) precision = SHOREFIT_PREC_FP32;
+        const bool mixed = precision != SHOREFIT_PREC_FP32;
         Design d;
-        upload_design(d, A, n_rows, n_cols, s);
+        upload_design(d, A, n_rows, n_cols, s, mixed);
         gram_and_lipschitz(d, s);
         if (lipschitz_out) *lipschitz_out = d.L;
         FistaOps ops;
@@ -315,6 +369,7 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         DevBuf<uint8_t> Cv(V);
         if (V) SF_HIP(hipMemcpyAsync(S.p, signals, sizeof(float) * V * n_rows, hipMemcpyHostToDevice, s));
         FistaLaunch L{};
+        FistaWork work;
         L.data = S.p;
         L.vox = nullptr;
         L.n_vox = V;
@@ -339,6 +394,8 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         L.resid = R.p;
         L.iters = It.p;
         L.converged = Cv.p;
+        setup_work(L, work, d, ops, V, s);
+        launch_fista_rhs(L, s);
         launch_fista(L, s);
         SF_HIP(hipStreamSynchronize(s));
         if (V) {
@@ -550,8 +607,9 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         if (out->rss_l2 && dev && nvox_total) SF_HIP(hipMemsetAsync(out->rss_l2, 0, sizeof(float) * nvox_total, s));
 
         // --- design + solver set-up (src/fit.rs:718-759) -------------------
+        const int precision = (l1 && sp.max_iter > 0) ? sp.precision : SHOREFIT_PREC_FP32;
         Design d;
-        build_design(d, *grads, big, small, sp.radial_order, sp.zeta, s);
+        build_design(d, *grads, big, small, sp.radial_order, sp.zeta, s, precision != SHOREFIT_PREC_FP32);
         hipEvent_t e0, e1;
         SF_HIP(hipEventCreate(&e0));
         SF_HIP(hipEventCreate(&e1));
@@ -559,8 +617,9 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             gram_and_lipschitz(d, s);
             out->lipschitz = d.L;
             FistaOps ops;
-            pack_fista(d, ops, sp.precision, s);
+            pack_fista(d, ops, precision, s);
             FistaLaunch L{};
+            FistaWork work;
             L.data = data;
             L.vox = d_vox.p;
             L.n_vox = V;
@@ -579,7 +638,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             L.max_iter = sp.max_iter;
             L.tol = float(sp.tol);
             L.nonneg = sp.non_negative;
-            L.precision = sp.precision;
+            L.precision = precision;
             L.refresh = std::max(1, sp.refresh_every);
             L.coef = coef;
             L.resid = res;
@@ -588,9 +647,12 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             L.kind = kind;
             L.alpha_map = amap;
             L.bic = bic;
+            setup_work(L, work, d, ops, V, s);
+            launch_fista_rhs(L, s);
             SF_HIP(hipEventRecord(e0, s));
+            L.event_mid = e1;  // the mixed path records e1 between solve and epilogue
             launch_fista(L, s);
-            SF_HIP(hipEventRecord(e1, s));
+            if (precision == SHOREFIT_PREC_FP32 || V == 0) SF_HIP(hipEventRecord(e1, s));
         } else {
             d.G.alloc(size_t(d.K) * d.K);
             launch_gram(d.A.p, d.N, d.K, d.G.p, s);

@@ -68,7 +68,7 @@ void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s);
 // ------------------------------------------------------------- packing ----
 // Fragment-major FP32 operands of the 16x16x4 f32 MFMA (layout described in
 // fista.hip). Padded rows/cols are zero.
-//   M  = I - c * G            (Kp x Kp), c = step / N
+//   M  = -c * G               (Kp x Kp), c = step / N (identity applied in-kernel)
 //   AT = c * A^T              (Kp x Np)
 //   A                          (Np x Kp)
 void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hipStream_t s);
@@ -91,6 +91,10 @@ struct FistaLaunch {
     const float* ATfrag;
     const float* Afrag;
     const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
+    const double* A64;      // FP64 design (N x K row-major) for the right-hand sides
+    double c;               // step / N
+    float* Bbuf;            // scratch [n_vox][Kp]: b = c A^T s
+    float* meanbuf;         // scratch [n_vox]: signal means
     float thr;              // step * alpha (fixed alpha)
     float step;             // 1 / L
     float ratio;            // alpha-ratio strategy: alpha = ratio * alpha_max (0 = fixed)
@@ -109,9 +113,18 @@ struct FistaLaunch {
     uint8_t* kind;          // optional
     float* alpha_map;       // optional
     float* bic;             // optional
-    // persistent-kernel work counter (mixed precision)
-    unsigned int* counter;
+    // mixed precision (persistent kernel) only
+    const float* mom;       // momentum table [max_iter]
+    unsigned int* counter;  // work counter
+    float* Xbuf;            // [n_vox][Kp]
+    uint32_t* itsbuf;       // [n_vox]
+    uint8_t* statusbuf;     // [n_vox]
+    float* alphabuf;        // [n_vox]
+    void* event_mid;        // optional hipEvent_t recorded between solve and epilogue
 };
+// b = c A^T s and signal means into L.Bbuf / L.meanbuf (FP64 accumulation).
+void launch_fista_rhs(const FistaLaunch& L, hipStream_t s);
+// The FISTA solve proper (reads L.Bbuf / L.meanbuf).
 void launch_fista(const FistaLaunch& L, hipStream_t s);
 
 struct TikhonovLaunch {

@@ -49,6 +49,9 @@ struct FistaArgs {
     const floatx4* Mfrag;
     const floatx4* ATfrag;
     const floatx4* Afrag;
+    const float* B;       // [n_vox][Kp] right-hand sides b = c A^T s (FP64-accumulated)
+    const float* mean;    // [n_vox] signal means (FP64-accumulated)
+    int Kp;
     float thr, step, ratio;
     double alpha;
     uint32_t max_iter;
@@ -157,6 +160,60 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
     }
 }
 
+// Right-hand sides b_v = c A^T s_v (c = 1/(N L)) and signal means, with
+// FP64 accumulation: b enters every FISTA iteration, so its rounding is a
+// perturbation of the problem itself; accumulating the N-term dot products
+// in FP32 measurably moved the FP32 trajectories away from the FP64 ones.
+// Workgroup = 64 voxels x 4 waves; wave w accumulates coefficients
+// k = w, w + 4, ... (wave-uniform, so A is read with scalar loads) and the
+// 64 lanes are the voxels, whose signal chunks are staged in LDS.
+template <int KQ>  // KQ = Kp / 4 coefficients per wave
+__global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data, const int64_t* __restrict__ vox,
+                                                  int64_t n_vox, int64_t sig_stride, int N, int K, int Kp,
+                                                  const double* __restrict__ A, double c, float* __restrict__ B,
+                                                  float* __restrict__ mean) {
+    constexpr int CH = 64;  // samples per LDS chunk
+    __shared__ float sm[64][CH + 1];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t v0 = int64_t(blockIdx.x) * 64;
+    double acc[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) acc[q] = 0.0;
+    double sum = 0.0;
+    for (int n0 = 0; n0 < N; n0 += CH) {
+        __syncthreads();
+        // cooperative load: 64 voxels x CH samples, each wave loads 16 voxels
+        for (int t = threadIdx.x; t < 64 * CH; t += 256) {
+            const int vj = t / CH, nn = t % CH;
+            const int64_t v = v0 + vj;
+            float x = 0.f;
+            if (v < n_vox && n0 + nn < N) x = data[(vox ? vox[v] : v) * sig_stride + n0 + nn];
+            sm[vj][nn] = x;
+        }
+        __syncthreads();
+        const int nmax = min(CH, N - n0);
+        for (int nn = 0; nn < nmax; ++nn) {
+            const double sv = double(sm[lane][nn]);
+            sum += sv;
+            const double* arow = A + size_t(n0 + nn) * K;
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) {
+                const int k = w + 4 * q;
+                if (k < K) acc[q] = fma(arow[k], sv, acc[q]);
+            }
+        }
+    }
+    const int64_t v = v0 + lane;
+    if (v >= n_vox) return;
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) {
+        const int k = w + 4 * q;
+        if (k < Kp) B[v * Kp + k] = k < K ? float(c * acc[q]) : 0.f;
+    }
+    if (w == 0) mean[v] = float(sum / double(N));
+}
+
 // Plain FP32 form: V = M Y + B recomputed exactly on FP32 MFMA every
 // iteration. WAVES waves per workgroup, 16 voxels per wave.
 template <int MB, int WAVES, bool M_IN_LDS>
@@ -179,9 +236,11 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
     const float* sig = a.data + out_idx * a.sig_stride;
 
     floatx4 b[MB];
-    float sum;
-    load_rhs<MB>(a, sig, valid, h, lane, b, sum);
-    const float mean = lane_sum4(sum) / float(a.N);
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+        b[mb] = valid ? *reinterpret_cast<const floatx4*>(a.B + v * a.Kp + 16 * mb + 4 * h)
+                      : floatx4{0.f, 0.f, 0.f, 0.f};
+    const float mean = valid ? a.mean[v] : 0.f;
 
     // alpha-ratio strategy: alpha_v = ratio * |A^T s|_inf / N, i.e.
     // thr_v = ratio * max|b| (src/solver/alpha.rs:229-235).
@@ -211,17 +270,21 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
 
     for (uint32_t it = 0; it < a.max_iter; ++it) {
         if (__ballot(active) == 0ull) break;
-        // One accumulation chain per 16-row block; back-to-back MFMAs that
-        // accumulate into the same registers issue without extra waits.
+        // An opaque zero offset keeps the (loop-invariant) M fragment loads
+        // inside the loop instead of hoisted into registers for all blocks.
+        int moff = 0;
+        asm volatile("" : "+s"(moff));
+        // acc = b - c G y on MFMA, one accumulation chain per 16-row block
+        // (back-to-back MFMAs into the same registers issue without extra
+        // waits); v = y + acc adds the identity term outside the chain so the
+        // FP32 partial sums stay small.
         floatx4 acc[MB];
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
-            // keep the scheduler from hoisting every block's M loads at once
-            asm volatile("" ::: "memory");
             floatx4 c = b[mb];
 #pragma unroll
             for (int kb = 0; kb < MB; ++kb) {
-                const floatx4 mf = load_m((mb * MB + kb) * 64 + lane);
+                const floatx4 mf = load_m(moff + (mb * MB + kb) * 64 + lane);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
             }
@@ -235,7 +298,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float val = acc[mb][r];
+                const float val = y[mb][r] + acc[mb][r];
                 float xn = val - fminf(fmaxf(val, -thr), thr);  // strict soft threshold
                 if (a.nonneg) xn = fmaxf(xn, 0.f);
                 const float d = xn - x[mb][r];
@@ -307,6 +370,14 @@ void launch_tik(const FistaArgs& args, int64_t n_vox, float* rss_l2, hipStream_t
 }
 
 template <int MB>
+void launch_rhs(const FistaLaunch& L, hipStream_t s) {
+    const unsigned grid = unsigned((L.n_vox + 63) / 64);
+    hipLaunchKernelGGL(rhs_kernel<4 * MB>, dim3(grid), dim3(256), 0, s, L.data, L.vox, L.n_vox, L.sig_stride, L.N,
+                       L.K, L.Kp, L.A64, L.c, L.Bbuf, L.meanbuf);
+    SF_HIP(hipGetLastError());
+}
+
+template <int MB>
 void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
     constexpr int WAVES = (MB >= 16) ? 4 : 8;
     constexpr bool IN_LDS = (MB * MB * 64 * 16) <= 112 * 1024;
@@ -321,7 +392,308 @@ void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
     SF_HIP(hipGetLastError());
 }
 
+
+// ---------------------------------------------------------------------------
+// Mixed-precision persistent form.
+//
+// The state of a voxel is (x, y, acc) with acc = b - c G y. Between exact
+// refreshes acc is advanced by the increment -c G (y+ - y) computed on the
+// 16x16x32 FP16 (or BF16) MFMA with FP32 accumulation; because the increment
+// shrinks as the iterates converge, its 11-bit (8-bit) operand rounding is
+// relative to |dy|, not to |y|. Every `refresh` iterations acc is recomputed
+// exactly as b - c G y on the FP32 MFMA (FP32 -cG read from L2), which bounds
+// the drift. dy is scaled per voxel by a power of two before the FP16
+// conversion: |dy| <= (1 + m_k)|d_k| + m_{k-1}|d_{k-1}| (d = x+ - x, m the
+// momentum), so 2^e |dy| < 2^14 never overflows.
+//
+// Persistent scheduling with active-voxel compaction: a wave owns 16 slots;
+// when a slot's voxel converges or reaches max_iter its solution is written
+// out and the slot is refilled with the next voxel from a global counter, so
+// MFMA tiles stay full while iteration counts vary from voxel to voxel.
+// Residual, R^2 and BIC are computed afterwards by a batched epilogue.
+// ---------------------------------------------------------------------------
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct MixedArgs {
+    const int64_t* vox;
+    int64_t n_vox;
+    int K, Kp;
+    const float* B;         // [n_vox][Kp]
+    const floatx4* M32;     // -cG, 16x16x4 f32 fragments (MB*MB*64)
+    const uint4* M16;       // -cG, 16x16x32 f16/bf16 fragments (MB*KC*64)
+    const float* mom;       // momentum (t_k - 1)/t_{k+1}, k < max_iter
+    float thr, step, ratio, alpha_fixed;
+    uint32_t max_iter;
+    float tol;
+    int nonneg;
+    int refresh;
+    unsigned int* counter;  // work counter, zero at launch
+    float* X;               // [n_vox][Kp] solutions, work order
+    uint32_t* its;          // [n_vox]
+    uint8_t* status;        // [n_vox] bit0 converged, bit1 all-zero voxel (alpha-ratio)
+    float* alpha_v;         // [n_vox] alpha used
+};
+
+template <bool BF16>
+__device__ __forceinline__ floatx4 mfma16(uint4 a, uint4 b, floatx4 c) {
+    if constexpr (BF16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                       c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a), __builtin_bit_cast(half8, b), c,
+                                                      0, 0, 0);
+}
+
+template <bool BF16>
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+    if constexpr (BF16) {
+        const __bf16 a = (__bf16)lo, b = (__bf16)hi;
+        return uint32_t(__builtin_bit_cast(uint16_t, a)) | (uint32_t(__builtin_bit_cast(uint16_t, b)) << 16);
+    } else {
+        const _Float16 a = (_Float16)lo, b = (_Float16)hi;
+        return uint32_t(__builtin_bit_cast(uint16_t, a)) | (uint32_t(__builtin_bit_cast(uint16_t, b)) << 16);
+    }
+}
+
+// B_LDS: keep each slot's b in LDS (needed by the refresh); otherwise in
+// registers (order 10, where FP16 -cG alone takes 128 KB of LDS).
+template <int MB, int WAVES, bool BF16, bool B_LDS>
+__global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs a) {
+    constexpr int KC = MB / 2;
+    constexpr int BSTRIDE = 16 * MB + 4;  // padded slot stride (floats) against LDS bank conflicts
+    extern __shared__ uint4 smem[];
+    uint4* Ml = smem;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    float* bsl = reinterpret_cast<float*>(smem + MB * KC * 64) + (wave * 16 + j) * BSTRIDE + 4 * h;
+    for (int i = threadIdx.x; i < MB * KC * 64; i += 64 * WAVES) Ml[i] = a.M16[i];
+    __syncthreads();
+
+    floatx4 x[MB], y[MB], acc[MB], breg[MB];
+    int64_t v = -1;
+    uint32_t it = 0;
+    float thr = a.thr, mprev = 0.f, dprev = 0.f;
+    bool alive = false, zero = false;
+
+    // Claims new voxels for the slots flagged in `need` (one atomic per
+    // wave) and loads them: x = y = 0, acc = b. An all-zero voxel of the
+    // alpha-ratio strategy (max|b| = 0) is run like any other; it converges
+    // at its first iteration with x = 0 and the epilogue reports it the way
+    // src/solver/alpha.rs:178-189 does.
+    auto refill = [&](bool need) {
+        const unsigned long long bal = __ballot(need && h == 0);
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(a.counter, unsigned(__popcll(bal)));
+        base = __shfl(base, 0);
+        if (need) {
+            v = int64_t(base) + __popcll(bal & ((1ull << j) - 1ull));
+            alive = v < a.n_vox;
+            it = 0;
+            mprev = dprev = 0.f;
+        }
+        float m = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            floatx4 bq = floatx4{0.f, 0.f, 0.f, 0.f};
+            if (need && alive) bq = *reinterpret_cast<const floatx4*>(a.B + v * a.Kp + 16 * mb + 4 * h);
+            if (need) {
+                x[mb] = y[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc[mb] = bq;
+                if constexpr (B_LDS)
+                    *reinterpret_cast<floatx4*>(bsl + 16 * mb) = bq;
+                else
+                    breg[mb] = bq;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) m = fmaxf(m, fabsf(bq[r]));
+        }
+        m = lane_max4(m);  // alpha-ratio: thr_v = ratio * max|b| (src/solver/alpha.rs:229-235)
+        if (need && a.ratio > 0.f) thr = a.ratio * m;
+        if (need) zero = alive && a.ratio > 0.f && m == 0.f;
+    };
+    refill(true);
+    uint32_t wit = 0;
+    const float tol = a.tol;
+    for (;;) {
+        if (__ballot(alive) == 0ull) break;
+        // An opaque zero offset keeps the (loop-invariant) -cG fragment loads
+        // inside the loop instead of hoisted into registers for all blocks.
+        int moff = 0;
+        asm volatile("" : "+s"(moff));
+        const float mom = alive ? a.mom[it] : 0.f;
+        float dd = 0.f, xx = 0.f;
+        floatx4 dy[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float val = y[mb][r] + acc[mb][r];
+                float xn = val - fminf(fmaxf(val, -thr), thr);
+                if (a.nonneg) xn = fmaxf(xn, 0.f);
+                const float d = xn - x[mb][r];
+                const float yn = fmaf(mom, d, xn);
+                dd = fmaf(d, d, dd);
+                xx = fmaf(xn, xn, xx);
+                dy[mb][r] = alive ? yn - y[mb][r] : 0.f;
+                if (alive) {
+                    x[mb][r] = xn;
+                    y[mb][r] = yn;
+                }
+            }
+        dd = lane_sum4(dd);
+        xx = lane_sum4(xx);
+        bool done = false, conv = false;
+        if (alive) {
+            ++it;
+            const float rel = sqrtf(dd) / fmaxf(sqrtf(xx), 1e-12f);
+            conv = rel < tol;
+            done = conv || it >= a.max_iter;
+        }
+        const float nd = sqrtf(dd);
+        const float bound = (1.f + mom) * nd + mprev * dprev;
+        mprev = mom;
+        dprev = nd;
+        ++wit;
+        if (wit % uint32_t(a.refresh) == 0) {
+            // exact refresh: acc = b - c G y on the FP32 MFMA
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                floatx4 c;
+                if constexpr (B_LDS)
+                    c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
+                else
+                    c = breg[mb];
+#pragma unroll
+                for (int kb = 0; kb < MB; ++kb) {
+                    const floatx4 mf = a.M32[moff + (mb * MB + kb) * 64 + lane];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+                }
+                acc[mb] = c;
+            }
+        } else {
+            int e = 0;
+            (void)frexpf(bound, &e);
+            const float sc = bound > 0.f ? ldexpf(1.f, 14 - e) : 1.f;
+            const float isc = bound > 0.f ? ldexpf(1.f, e - 14) : 1.f;
+            uint4 dyh[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) {
+                const floatx4 p = dy[2 * kc] * sc, q = dy[2 * kc + 1] * sc;
+                dyh[kc] = uint4{pack2<BF16>(p[0], p[1]), pack2<BF16>(p[2], p[3]), pack2<BF16>(q[0], q[1]),
+                                pack2<BF16>(q[2], q[3])};
+            }
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                floatx4 c = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kc = 0; kc < KC; ++kc) c = mfma16<BF16>(Ml[moff + (mb * KC + kc) * 64 + lane], dyh[kc], c);
+                acc[mb] += c * isc;
+            }
+        }
+        if (__ballot(done) != 0ull) {
+            if (done) {
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+                    *reinterpret_cast<floatx4*>(a.X + v * a.Kp + 16 * mb + 4 * h) = x[mb];
+                if (h == 0) {
+                    a.its[v] = zero ? 0u : it;
+                    a.status[v] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+                    a.alpha_v[v] = zero ? 0.f : (a.ratio > 0.f ? thr / a.step : a.alpha_fixed);
+                }
+            }
+            refill(done);
+        }
+    }
+}
+
+// Batched epilogue of the mixed form: residual, R^2, BIC, scatter.
+template <int MB>
+__global__ void __launch_bounds__(256) fista_epilogue_kernel(const FistaArgs a, const float* X, const uint32_t* its,
+                                                             const uint8_t* status, const float* alpha_v) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    const int64_t v = (int64_t(blockIdx.x) * 4 + wave) * 16 + j;
+    const bool valid = v < a.n_vox;
+    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
+    const float* sig = a.data + out_idx * a.sig_stride;
+    floatx4 x[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+        x[mb] = valid ? *reinterpret_cast<const floatx4*>(X + v * a.Kp + 16 * mb + 4 * h)
+                      : floatx4{0.f, 0.f, 0.f, 0.f};
+    const float mean = valid ? a.mean[v] : 0.f;
+    float rss, sst;
+    residual<MB>(a, sig, valid, h, lane, x, mean, rss, sst);
+    if (!valid) return;
+    const uint8_t st = status[v];
+    write_voxel<MB>(a, out_idx, h, x, rss, sst, its[v], st & 1, double(alpha_v[v]), (st & 2) != 0);
+}
+
+template <int MB, bool BF16>
+void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
+    constexpr bool B_LDS = MB <= 12;
+    constexpr int WAVES = B_LDS ? 8 : 4;
+    constexpr int KC = MB / 2;
+    MixedArgs m{};
+    m.vox = L.vox;
+    m.n_vox = L.n_vox;
+    m.K = L.K;
+    m.Kp = L.Kp;
+    m.B = L.Bbuf;
+    m.M32 = reinterpret_cast<const floatx4*>(L.Mfrag);
+    m.M16 = reinterpret_cast<const uint4*>(L.Mlow);
+    m.mom = L.mom;
+    m.thr = L.thr;
+    m.step = L.step;
+    m.ratio = L.ratio;
+    m.alpha_fixed = float(L.alpha);
+    m.max_iter = L.max_iter;
+    m.tol = L.tol;
+    m.nonneg = L.nonneg;
+    m.refresh = L.refresh < 1 ? 1 : L.refresh;
+    m.counter = L.counter;
+    m.X = L.Xbuf;
+    m.its = L.itsbuf;
+    m.status = L.statusbuf;
+    m.alpha_v = L.alphabuf;
+    const size_t lds = size_t(MB) * KC * 64 * 16 + (B_LDS ? size_t(WAVES) * 16 * (16 * MB + 4) * 4 : 0);
+    auto kern = fista_mixed_kernel<MB, WAVES, BF16, B_LDS>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(lds));
+    int dev = 0, cus = 0, per_cu = 0;
+    SF_HIP(hipGetDevice(&dev));
+    SF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 64 * WAVES,
+                                                        lds));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t need = (L.n_vox + 16 * WAVES - 1) / (16 * WAVES);
+    const int64_t grid = std::min<int64_t>(need, int64_t(cus) * per_cu);
+    SF_HIP(hipMemsetAsync(L.counter, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(64 * WAVES), lds, s, m);
+    SF_HIP(hipGetLastError());
+    if (L.event_mid) SF_HIP(hipEventRecord(static_cast<hipEvent_t>(L.event_mid), s));
+    const int64_t tiles = (L.n_vox + 15) / 16;
+    hipLaunchKernelGGL(fista_epilogue_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, ea, L.Xbuf,
+                       L.itsbuf, L.statusbuf, L.alphabuf);
+    SF_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+void launch_fista_rhs(const FistaLaunch& L, hipStream_t s) {
+    if (L.n_vox <= 0) return;
+    switch (L.Kp / 16) {
+#define SF_RHS(MB) \
+    case MB: launch_rhs<MB>(L, s); break;
+        SF_RHS(1) SF_RHS(2) SF_RHS(3) SF_RHS(4) SF_RHS(5) SF_RHS(6) SF_RHS(8) SF_RHS(10) SF_RHS(12) SF_RHS(16)
+#undef SF_RHS
+        default:
+            throw Error{SHOREFIT_E_UNSUPPORTED,
+                        "right-hand-side kernel: padded coefficient count " + std::to_string(L.Kp) + " is not instantiated"};
+    }
+}
 
 void launch_fista(const FistaLaunch& L, hipStream_t s) {
     if (L.n_vox <= 0) return;
@@ -351,6 +723,27 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
     a.kind = L.kind;
     a.alpha_map = L.alpha_map;
     a.bic = L.bic;
+    a.B = L.Bbuf;
+    a.mean = L.meanbuf;
+    a.Kp = L.Kp;
+    if (L.precision != SHOREFIT_PREC_FP32) {
+        const bool bf = L.precision == SHOREFIT_PREC_BF16;
+        switch (L.Kp / 16) {
+#define SF_MIX(MB)                                                    \
+    case MB:                                                          \
+        if (bf)                                                       \
+            launch_mixed_mb<MB, true>(L, a, s);                       \
+        else                                                          \
+            launch_mixed_mb<MB, false>(L, a, s);                      \
+        break;
+            SF_MIX(2) SF_MIX(4) SF_MIX(6) SF_MIX(8) SF_MIX(10) SF_MIX(16)
+#undef SF_MIX
+            default:
+                throw Error{SHOREFIT_E_UNSUPPORTED, "mixed-precision FISTA: padded coefficient count " +
+                                                        std::to_string(L.Kp) + " is not instantiated"};
+        }
+        return;
+    }
     switch (L.Kp / 16) {
         case 1: launch_f32<1>(a, L.n_vox, s); break;
         case 2: launch_f32<2>(a, L.n_vox, s); break;

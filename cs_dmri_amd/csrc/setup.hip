@@ -231,7 +231,8 @@ __global__ void pack_rowblock_kernel(const double* __restrict__ src, int rows, i
     frag[t] = float(v);
 }
 
-// M = I - c G in the same fragment order, zero outside K x K.
+// -c G in the same fragment order, zero outside K x K (the FISTA kernels add
+// the identity term of M = I - c G outside the MFMA chain).
 __global__ void pack_M_kernel(const double* __restrict__ G, int K, int Kp, double c, float* __restrict__ frag) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int MB = Kp / 16;
@@ -241,11 +242,11 @@ __global__ void pack_M_kernel(const double* __restrict__ G, int K, int Kp, doubl
     const int rb = int(blk / MB), cb = int(blk % MB);
     const int i = 16 * rb + (l & 15), j = 16 * cb + 4 * (l >> 4) + r;
     double v = 0.0;
-    if (i < K && j < K) v = (i == j ? 1.0 : 0.0) - c * G[size_t(i) * K + j];
+    if (i < K && j < K) v = -c * G[size_t(i) * K + j];
     frag[t] = float(v);
 }
 
-// M = I - c G as the A operand of the 16x16x32 FP16/BF16 MFMA: for output
+// -c G as the A operand of the 16x16x32 FP16/BF16 MFMA: for output
 // block mb, k-chunk kc (32 rows of Y, i.e. 16-row blocks 2kc and 2kc+1) and
 // lane l = 16h + i, eight values M[16mb + i][p(v)], v = 0..7, where
 // p(v) = 32kc + 16(v/4) + 4h + v%4 is the row of Y that lane (h, j) of the
@@ -261,7 +262,7 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
     const int h = l >> 4, i = 16 * mb + (l & 15);
     const int p = 32 * kc + 16 * (v >> 2) + 4 * h + (v & 3);
     double x = 0.0;
-    if (i < K && p < K) x = (i == p ? 1.0 : 0.0) - c * G[size_t(i) * K + p];
+    if (i < K && p < K) x = -c * G[size_t(i) * K + p];
     if (bf16) {
         __hip_bfloat16 b = __float2bfloat16(float(x));
         frag[t] = *reinterpret_cast<uint16_t*>(&b);

@@ -1,0 +1,110 @@
+"""GPU parity of the mixed-precision persistent FISTA kernel.
+
+FP16 (or BF16) MFMA increments of acc = b - cG y with an exact FP32 refresh
+every `refresh_every` iterations, per-voxel power-of-two scaling, and
+persistent slots refilled from a work counter.
+
+Tolerances (coefficients relative to the FP64 oracle, per voxel):
+* FP16, refresh 10, equal iteration counts (tol = 0), 500 iterations:
+  max <= 1e-4 (the north-star bound for the mixed-precision fit).
+* BF16, refresh 10, same: max <= 1e-3 (8-bit operand mantissa).
+* With the stopping rule: median <= 1e-4, p99 <= 1e-3.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import cs_dmri_amd as sf  # noqa: E402
+from cs_dmri_amd import workloads as W  # noqa: E402
+
+
+def _rel(c, ref):
+    return np.linalg.norm(c - ref, axis=-1) / np.maximum(np.linalg.norm(ref, axis=-1), 1e-30)
+
+
+@pytest.fixture(scope="module")
+def cfg1(orc):
+    wl = W.config1()
+    g = wl.gtab
+    A = orc.design_matrix(6, 700.0, g.bvals, g.bvecs, g.big_delta, g.small_delta)
+    S = wl.data[wl.mask]
+    ref500 = orc.fit_volume(S[:512].reshape(512, 1, 1, -1), None, A, alpha_mode="fixed", alpha=1e-3, max_iter=500,
+                            tol=0.0)["coefficients"].reshape(512, -1).astype(np.float64)
+    return wl, A, S, ref500
+
+
+@pytest.mark.parametrize("precision,refresh,bound", [("fp16", 10, 1e-4), ("fp16", 5, 1e-4), ("bf16", 10, 1e-3)])
+def test_mixed_equal_iterations(cfg1, precision, refresh, bound):
+    wl, A, S, ref = cfg1
+    coef, d = sf.fista_batch(A, S[:512], 1e-3, max_iter=500, tol=0.0, precision=precision, refresh_every=refresh)
+    assert (d["iterations"] == 500).all() and not d["converged"].any()
+    err = _rel(coef.astype(np.float64), ref)
+    print(f"{precision} k={refresh}: max {err.max():.2e} median {np.median(err):.2e}")
+    assert err.max() <= bound
+
+
+def test_mixed_stopping_rule_volume(orc, cfg1):
+    wl, A, S, _ = cfg1
+    kw = dict(big_delta=wl.gtab.big_delta, small_delta=wl.gtab.small_delta, radial_order=6, alpha_mode="fixed",
+              alpha=1e-3, max_iter=500, tol=1e-6, diagnostics=True)
+    out = sf.shore_fit(wl.data, wl.gtab.bvals, wl.gtab.bvecs, wl.mask, precision="fp16", **kw)
+    ref = orc.fit_volume(wl.data, wl.mask, A, alpha_mode="fixed", alpha=1e-3, max_iter=500, tol=1e-6)
+    err = _rel(out["coefficients"][wl.mask].astype(np.float64), ref["coefficients"][wl.mask].astype(np.float64))
+    print(f"fp16 tol=1e-6: median {np.median(err):.2e} p99 {np.quantile(err, 0.99):.2e} max {err.max():.2e}")
+    assert np.median(err) <= 1e-4 and np.quantile(err, 0.99) <= 1e-3
+    it, itr = out["iterations"][wl.mask].astype(float), ref["iterations"][wl.mask].astype(float)
+    assert np.mean(np.abs(it - itr) <= 0.1 * itr) >= 0.9
+    np.testing.assert_allclose(out["residual_l2"][wl.mask], ref["residual_l2"][wl.mask], rtol=2e-3)
+    assert (out["regularization_kind"][wl.mask] == 1).all() and np.all(out["alpha"][wl.mask] == np.float32(1e-3))
+
+
+def test_mixed_uneven_convergence_matches_fp32():
+    # Mixed SNR (5..40) makes iteration counts uneven, exercising slot refills.
+    gt = W.cartesian_table(100, seed=1)
+    S = W.mixed_snr_signals(gt, 3000, seed=5)
+    A = sf.design_matrix(gt.bvals, gt.bvecs, radial_order=6, big_delta=gt.big_delta, small_delta=gt.small_delta)
+    c32, d32 = sf.fista_batch(A, S, 1e-3, max_iter=400, tol=1e-6)
+    c16, d16 = sf.fista_batch(A, S, 1e-3, max_iter=400, tol=1e-6, precision="fp16")
+    assert d32["iterations"].min() < 100 and d32["iterations"].max() >= 300  # uneven workload
+    err = _rel(c16.astype(np.float64), c32.astype(np.float64))
+    assert np.median(err) <= 1e-4 and np.quantile(err, 0.99) <= 1e-3
+    assert np.mean(np.abs(d16["iterations"].astype(int) - d32["iterations"].astype(int))
+                   <= 0.1 * d32["iterations"]) >= 0.9
+    np.testing.assert_allclose(d16["residual_l2"], d32["residual_l2"], rtol=2e-3)
+
+
+def test_mixed_alpha_ratio_and_zero_voxels(orc):
+    gt = W.cartesian_table(60, seed=21)
+    shape = (6, 5, 4)
+    mask = np.ones(shape, bool)
+    mask[0, 0, :] = False
+    data = np.zeros(shape + (len(gt.bvals),), np.float32)
+    data[mask] = W.multi_tensor_signal(gt, int(mask.sum()), W.S0 / 30, seed=22)
+    data[1, 2, 0] = data[3, 3, 3] = 0.0
+    kw = dict(big_delta=gt.big_delta, small_delta=gt.small_delta, radial_order=4, alpha_mode="alpha-ratio",
+              alpha_ratio=1e-2, max_iter=300, tol=0.0, diagnostics=True)
+    out = sf.shore_fit(data, gt.bvals, gt.bvecs, mask, precision="fp16", **kw)
+    A = orc.design_matrix(4, 700.0, gt.bvals, gt.bvecs, gt.big_delta, gt.small_delta)
+    ref = orc.fit_volume(data, mask, A, alpha_mode="alpha-ratio", alpha_ratio=1e-2, max_iter=300, tol=0.0)
+    np.testing.assert_array_equal(out["iterations"], ref["iterations"])
+    for z in [(1, 2, 0), (3, 3, 3)]:
+        assert out["iterations"][z] == 0 and out["alpha"][z] == 0 and out["bic"][z] == 0
+        assert not out["coefficients"][z].any() and out["residual_l2"][z] == 0
+    np.testing.assert_allclose(out["alpha"], ref["alpha"], rtol=1e-5)
+    live = np.linalg.norm(ref["coefficients"], axis=-1) > 0
+    err = _rel(out["coefficients"][live].astype(float), ref["coefficients"][live].astype(float))
+    assert err.max() <= 1e-4
+    assert not out["coefficients"][~mask].any()
+
+
+def test_mixed_order8_against_oracle_sample(orc):
+    gt, S = W.config2_signals(256, seed=7)
+    A = orc.design_matrix(8, 700.0, gt.bvals, gt.bvecs, gt.big_delta, gt.small_delta)
+    c16, d16 = sf.fista_batch(A, S, 1e-3, max_iter=500, tol=0.0, precision="fp16")
+    ref = orc.fit_volume(S.reshape(256, 1, 1, -1), None, A, alpha_mode="fixed", alpha=1e-3, max_iter=500, tol=0.0,
+                         n_threads=8)["coefficients"].reshape(256, -1).astype(np.float64)
+    err = _rel(c16.astype(np.float64), ref)
+    print(f"order 8 fp16: max {err.max():.2e} median {np.median(err):.2e}")
+    assert err.max() <= 1e-4
