@@ -160,7 +160,8 @@ void gram_and_lipschitz(Design& d, hipStream_t s) {
 }
 
 struct FistaOps {
-    DevBuf<float> M, AT, A;
+    DevBuf<float> M, A;
+    DevBuf<double> Ap;  // FP64 design zero-padded to N x Kp
     DevBuf<uint16_t> Mlow;
     double step = 0.0;
 };
@@ -172,6 +173,8 @@ void pack_fista(const Design& d, FistaOps& ops, int precision, hipStream_t s) {
     ops.A.alloc(size_t(d.Np) * d.Kp);
     launch_pack_M(d.G.p, d.K, d.Kp, c, ops.M.p, s);
     launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
+    ops.Ap.alloc(size_t(d.N) * d.Kp);
+    launch_pad_cols(d.A.p, d.N, d.K, d.Kp, ops.Ap.p, s);
     if (precision != SHOREFIT_PREC_FP32) {
         ops.Mlow.alloc(size_t(d.Kp) * d.Kp);
         launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, precision == SHOREFIT_PREC_BF16, ops.Mlow.p, s);
@@ -191,7 +194,7 @@ struct FistaWork {
 void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& ops, int64_t V, hipStream_t s) {
     w.B.alloc(size_t(V) * d.Kp);
     w.mean.alloc(V);
-    L.A64 = d.A.p;
+    L.A64p = ops.Ap.p;
     L.c = ops.step / double(d.N);
     L.Bbuf = w.B.p;
     L.meanbuf = w.mean.p;
@@ -379,7 +382,6 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         L.Kp = d.Kp;
         L.Np = d.Np;
         L.Mfrag = ops.M.p;
-        L.ATfrag = ops.AT.p;
         L.Afrag = ops.A.p;
         L.step = float(ops.step);
         L.thr = float(ops.step * alpha);
@@ -629,7 +631,6 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             L.Kp = d.Kp;
             L.Np = d.Np;
             L.Mfrag = ops.M.p;
-            L.ATfrag = ops.AT.p;
             L.Afrag = ops.A.p;
             L.step = float(ops.step);
             L.thr = float(ops.step * sp.alpha);

@@ -76,6 +76,8 @@ void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, b
                           int rows_p, int cols_p, float* d_frag, hipStream_t s);
 // Same operand in FP16 / BF16 16x16x32 fragment order (mixed kernel).
 void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s);
+// dst (rows x cols_p, FP64) = src (rows x cols) zero-padded on the right.
+void launch_pad_cols(const double* d_src, int rows, int cols, int cols_p, double* d_dst, hipStream_t s);
 
 // ---------------------------------------------------------------- fits ----
 struct FistaLaunch {
@@ -91,7 +93,7 @@ struct FistaLaunch {
     const float* ATfrag;
     const float* Afrag;
     const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
-    const double* A64;      // FP64 design (N x K row-major) for the right-hand sides
+    const double* A64p;     // FP64 design zero-padded to N x Kp (right-hand sides)
     double c;               // step / N
     float* Bbuf;            // scratch [n_vox][Kp]: b = c A^T s
     float* meanbuf;         // scratch [n_vox]: signal means

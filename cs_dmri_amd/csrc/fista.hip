@@ -164,18 +164,19 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
 // FP64 accumulation: b enters every FISTA iteration, so its rounding is a
 // perturbation of the problem itself; accumulating the N-term dot products
 // in FP32 measurably moved the FP32 trajectories away from the FP64 ones.
-// Workgroup = 64 voxels x 4 waves; wave w accumulates coefficients
-// k = w, w + 4, ... (wave-uniform, so A is read with scalar loads) and the
-// 64 lanes are the voxels, whose signal chunks are staged in LDS.
+// Workgroup = 64 voxels x 4 waves; the 64 lanes are the voxels (signal
+// chunks staged in LDS) and wave w owns the contiguous coefficient range
+// [w KQ, (w+1) KQ) of the zero-padded design Ap (N x Kp, FP64), which it reads
+// with wave-uniform (scalar) loads.
 template <int KQ>  // KQ = Kp / 4 coefficients per wave
 __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data, const int64_t* __restrict__ vox,
-                                                  int64_t n_vox, int64_t sig_stride, int N, int K, int Kp,
-                                                  const double* __restrict__ A, double c, float* __restrict__ B,
+                                                  int64_t n_vox, int64_t sig_stride, int N, int Kp,
+                                                  const double* __restrict__ Ap, double c, float* __restrict__ B,
                                                   float* __restrict__ mean) {
     constexpr int CH = 64;  // samples per LDS chunk
     __shared__ float sm[64][CH + 1];
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int k0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * KQ;
     const int64_t v0 = int64_t(blockIdx.x) * 64;
     double acc[KQ];
 #pragma unroll
@@ -183,7 +184,6 @@ __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data
     double sum = 0.0;
     for (int n0 = 0; n0 < N; n0 += CH) {
         __syncthreads();
-        // cooperative load: 64 voxels x CH samples, each wave loads 16 voxels
         for (int t = threadIdx.x; t < 64 * CH; t += 256) {
             const int vj = t / CH, nn = t % CH;
             const int64_t v = v0 + vj;
@@ -196,22 +196,18 @@ __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data
         for (int nn = 0; nn < nmax; ++nn) {
             const double sv = double(sm[lane][nn]);
             sum += sv;
-            const double* arow = A + size_t(n0 + nn) * K;
+            const double* arow = Ap + size_t(n0 + nn) * Kp + k0;
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) {
-                const int k = w + 4 * q;
-                if (k < K) acc[q] = fma(arow[k], sv, acc[q]);
-            }
+            for (int q = 0; q < KQ; ++q) acc[q] = fma(arow[q], sv, acc[q]);
         }
     }
     const int64_t v = v0 + lane;
     if (v >= n_vox) return;
 #pragma unroll
-    for (int q = 0; q < KQ; ++q) {
-        const int k = w + 4 * q;
-        if (k < Kp) B[v * Kp + k] = k < K ? float(c * acc[q]) : 0.f;
-    }
-    if (w == 0) mean[v] = float(sum / double(N));
+    for (int q = 0; q < KQ; q += 4)
+        *reinterpret_cast<floatx4*>(B + v * Kp + k0 + q) =
+            floatx4{float(c * acc[q]), float(c * acc[q + 1]), float(c * acc[q + 2]), float(c * acc[q + 3])};
+    if (k0 == 0) mean[v] = float(sum / double(N));
 }
 
 // Plain FP32 form: V = M Y + B recomputed exactly on FP32 MFMA every
@@ -373,7 +369,7 @@ template <int MB>
 void launch_rhs(const FistaLaunch& L, hipStream_t s) {
     const unsigned grid = unsigned((L.n_vox + 63) / 64);
     hipLaunchKernelGGL(rhs_kernel<4 * MB>, dim3(grid), dim3(256), 0, s, L.data, L.vox, L.n_vox, L.sig_stride, L.N,
-                       L.K, L.Kp, L.A64, L.c, L.Bbuf, L.meanbuf);
+                       L.Kp, L.A64p, L.c, L.Bbuf, L.meanbuf);
     SF_HIP(hipGetLastError());
 }
 

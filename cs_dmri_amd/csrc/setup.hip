@@ -129,6 +129,30 @@ __global__ void gram_kernel(const double* __restrict__ A, int N, int K, double* 
     G[size_t(p) * K + q] = acc;
 }
 
+
+// S_out = S_in S_in / max|S_in S_in| for a symmetric K x K matrix (one
+// thread per entry; the scale keeps repeated squaring in range).
+__global__ void square_scaled_kernel(const double* __restrict__ S, int K, double* __restrict__ out) {
+    const int p = blockIdx.y, q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= K) return;
+    double acc = 0.0;
+    for (int i = 0; i < K; ++i) acc += S[size_t(p) * K + i] * S[size_t(i) * K + q];
+    out[size_t(p) * K + q] = acc;
+}
+
+__global__ void __launch_bounds__(1024) normalize_max_kernel(double* __restrict__ S, int n) {
+    __shared__ double red[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(S[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = 0.0;
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
+    const double inv = m > 0.0 ? 1.0 / m : 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) S[i] *= inv;
+}
+
 __device__ double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -140,22 +164,54 @@ __device__ double block_sum(double v, double* red) {
     return s;
 }
 
-// Power iteration on the symmetric PSD Gram matrix: lambda = v^T G v with
-// v <- G v / |G v|, stopped once the Rayleigh quotient has not moved by more
-// than 1e-16 relative for 16 consecutive sweeps (or after 200000 sweeps).
-// The dominant-eigenvalue gap of SHORE Gram matrices is ~0.86-0.94, so this
-// settles in a few hundred sweeps.
-__global__ void __launch_bounds__(256) power_iteration_kernel(const double* __restrict__ G, int K,
-                                                              double* __restrict__ out) {
+// Largest eigenvalue of the symmetric PSD Gram matrix G. The dominant
+// eigenvector is found by power iteration on P = G^16 (scaled), whose
+// eigenvalue ratio is (l2/l1)^16: SHORE Gram matrices have l2/l1 ~ 0.86-0.94,
+// so P's ratio is <= 0.37 and a few dozen sweeps pin the vector to ~1e-12.
+// The eigenvalue is then the Rayleigh quotient v^T G v / v^T v with G itself,
+// polished by power sweeps on G until it stops moving (its error is quadratic
+// in the eigenvector error, so it reaches the FP64 floor; it never exceeds
+// l1). This replaces the reference's full SVD (src/solver/fista.rs:45-51).
+__global__ void __launch_bounds__(1024) power_iteration_kernel(const double* __restrict__ P,
+                                                               const double* __restrict__ G, int K,
+                                                               double* __restrict__ out) {
     extern __shared__ double sm[];
     double* v = sm;
     double* w = sm + K;
-    __shared__ double red[8];
-    for (int i = threadIdx.x; i < K; i += blockDim.x) v[i] = 1.0 / sqrt(double(K)) * (1.0 + 1e-3 * i);
+    __shared__ double red[16];
+    for (int i = threadIdx.x; i < K; i += blockDim.x) v[i] = 1.0 + 1e-3 * i;
     __syncthreads();
+    // phase 1: eigenvector of P
+    for (int it = 0; it < 400; ++it) {
+        double nrm = 0.0, diff = 0.0;
+        for (int p = threadIdx.x; p < K; p += blockDim.x) {
+            double acc = 0.0;
+            for (int q = 0; q < K; ++q) acc += P[size_t(p) * K + q] * v[q];
+            w[p] = acc;
+            nrm += acc * acc;
+        }
+        const double ww = block_sum(nrm, red);
+        if (!(ww > 0.0)) break;
+        const double inv = 1.0 / sqrt(ww);
+        for (int p = threadIdx.x; p < K; p += blockDim.x) {
+            const double nv = w[p] * inv;
+            diff = fmax(diff, fabs(nv - v[p]));
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < K; p += blockDim.x) v[p] = w[p] * inv;
+        for (int o = 32; o > 0; o >>= 1) diff = fmax(diff, __shfl_xor(diff, o));
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = diff;
+        __syncthreads();
+        double dmax = 0.0;
+        for (int i = 0; i < int(blockDim.x >> 6); ++i) dmax = fmax(dmax, red[i]);
+        __syncthreads();
+        if (dmax < 1e-13) break;
+    }
+    // phase 2: Rayleigh quotient with G, polished by power sweeps on G
     double lam = 0.0;
     int stable = 0;
-    for (int it = 0; it < 200000 && stable < 16; ++it) {
+    for (int it = 0; it < 200 && stable < 2; ++it) {
         double part = 0.0, nrm = 0.0, vpart = 0.0;
         for (int p = threadIdx.x; p < K; p += blockDim.x) {
             double acc = 0.0;
@@ -168,14 +224,14 @@ __global__ void __launch_bounds__(256) power_iteration_kernel(const double* __re
         const double vv = block_sum(vpart, red);
         const double num = block_sum(part, red);
         const double ww = block_sum(nrm, red);
-        const double lam_new = num / vv;
-        stable = (fabs(lam_new - lam) <= 1e-16 * fabs(lam_new)) ? stable + 1 : 0;
-        lam = lam_new;
+        const double lam_new = vv > 0.0 ? num / vv : 0.0;
+        stable = (fabs(lam_new - lam) <= 1e-15 * fabs(lam_new)) ? stable + 1 : 0;
+        lam = fmax(lam, lam_new);
         const double inv = ww > 0.0 ? 1.0 / sqrt(ww) : 0.0;
         __syncthreads();
         for (int p = threadIdx.x; p < K; p += blockDim.x) v[p] = w[p] * inv;
         __syncthreads();
-        if (ww == 0.0) break;
+        if (!(ww > 0.0)) break;
     }
     if (threadIdx.x == 0) out[0] = lam;
 }
@@ -291,6 +347,14 @@ __global__ void chol_solve_columns_kernel(const double* __restrict__ Lf, int K, 
     }
 }
 
+__global__ void pad_cols_kernel(const double* __restrict__ src, int rows, int cols, int cols_p,
+                                double* __restrict__ dst) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(rows) * cols_p) return;
+    const int i = int(t / cols_p), j = int(t % cols_p);
+    dst[t] = j < cols ? src[size_t(i) * cols + j] : 0.0;
+}
+
 inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
@@ -310,9 +374,20 @@ void launch_gram(const double* d_A, int N, int K, double* d_G, hipStream_t s) {
 }
 
 void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStream_t s) {
-    hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(256), size_t(2 * K) * sizeof(double), s, d_G, K,
+    // P = G^16 by four scaled squarings (ping-pong buffers)
+    DevBuf<double> p0(size_t(K) * K), p1(size_t(K) * K);
+    const double* src = d_G;
+    double* bufs[2] = {p0.p, p1.p};
+    for (int i = 0; i < 4; ++i) {
+        double* dst = bufs[i & 1];
+        hipLaunchKernelGGL(square_scaled_kernel, dim3(blocks_for(K, 64), K), dim3(64), 0, s, src, K, dst);
+        hipLaunchKernelGGL(normalize_max_kernel, dim3(1), dim3(1024), 0, s, dst, K * K);
+        src = dst;
+    }
+    hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(1024), size_t(2 * K) * sizeof(double), s, src, d_G, K,
                        d_lambda);
     SF_HIP(hipGetLastError());
+    SF_HIP(hipStreamSynchronize(s));  // p0 / p1 are released at scope exit
 }
 
 void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s) {
@@ -343,6 +418,12 @@ void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, u
 
 void launch_chol_solve_columns(const double* d_L, int K, const double* d_A, int N, double* d_P, hipStream_t s) {
     hipLaunchKernelGGL(chol_solve_columns_kernel, dim3(blocks_for(N, 64)), dim3(64), 0, s, d_L, K, d_A, N, d_P);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_pad_cols(const double* d_src, int rows, int cols, int cols_p, double* d_dst, hipStream_t s) {
+    hipLaunchKernelGGL(pad_cols_kernel, dim3(blocks_for(int64_t(rows) * cols_p, 256)), dim3(256), 0, s, d_src, rows,
+                       cols, cols_p, d_dst);
     SF_HIP(hipGetLastError());
 }
 

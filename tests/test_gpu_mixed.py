@@ -67,7 +67,8 @@ def test_mixed_uneven_convergence_matches_fp32():
     A = sf.design_matrix(gt.bvals, gt.bvecs, radial_order=6, big_delta=gt.big_delta, small_delta=gt.small_delta)
     c32, d32 = sf.fista_batch(A, S, 1e-3, max_iter=400, tol=1e-6)
     c16, d16 = sf.fista_batch(A, S, 1e-3, max_iter=400, tol=1e-6, precision="fp16")
-    assert d32["iterations"].min() < 100 and d32["iterations"].max() >= 300  # uneven workload
+    its = d32["iterations"]
+    assert its.max() >= 2 * its.min() and its.max() >= 300  # uneven workload
     err = _rel(c16.astype(np.float64), c32.astype(np.float64))
     assert np.median(err) <= 1e-4 and np.quantile(err, 0.99) <= 1e-3
     assert np.mean(np.abs(d16["iterations"].astype(int) - d32["iterations"].astype(int))
