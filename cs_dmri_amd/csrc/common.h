@@ -74,7 +74,7 @@ void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s);
 void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hipStream_t s);
 void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, bool transpose, double scale,
                           int rows_p, int cols_p, float* d_frag, hipStream_t s);
-// Same operand in FP16 / BF16 16x16x32 fragment order (mixed kernel).
+// -c G in FP16 / BF16 16x16x32 fragment order (mixed kernel increments).
 void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s);
 // dst (rows x cols_p, FP64) = src (rows x cols) zero-padded on the right.
 void launch_pad_cols(const double* d_src, int rows, int cols, int cols_p, double* d_dst, hipStream_t s);

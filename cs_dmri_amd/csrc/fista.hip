@@ -392,24 +392,33 @@ void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Mixed-precision persistent form.
 //
-// The state of a voxel is (x, y, acc) with acc = b - c G y. Between exact
-// refreshes acc is advanced by the increment -c G (y+ - y) computed on the
-// 16x16x32 FP16 (or BF16) MFMA with FP32 accumulation; because the increment
-// shrinks as the iterates converge, its 11-bit (8-bit) operand rounding is
-// relative to |dy|, not to |y|. Every `refresh` iterations acc is recomputed
-// exactly as b - c G y on the FP32 MFMA (FP32 -cG read from L2), which bounds
-// the drift. dy is scaled per voxel by a power of two before the FP16
-// conversion: |dy| <= (1 + m_k)|d_k| + m_{k-1}|d_{k-1}| (d = x+ - x, m the
-// momentum), so 2^e |dy| < 2^14 never overflows.
+// Per voxel the kernel keeps x, y and acc = b - c G y, all scaled by a
+// per-voxel power of two 2^e; the pre-threshold value is y + acc. Between
+// exact refreshes acc advances by -c G dy on the 16x16x32 FP16 (or BF16)
+// MFMA, accumulating straight into acc in FP32 with the FP16 copy of -cG
+// staged in LDS. Only this -cG dy term sees the FP16 rounding of dy, and that
+// rounding is relative to |dy|, which shrinks as the iterate converges (acc,
+// not the larger y + acc, carries the FP32 accumulation, which keeps the
+// rounding noise under the stopping test small). Every `refresh` iterations
+// acc is recomputed exactly as b 2^e - c G y on the FP32 MFMA (FP32 -cG read
+// from L2), which bounds the drift, and the scale is re-chosen: 2^e puts the
+// bound |dy| <= (1 + m_k)|d_k| + m_{k-1}|d_{k-1}| near 2^10, so the FP16 copy
+// of dy has headroom; a slot whose bound would pass 2^15 forces an exact
+// refresh before it is converted. Scaling by powers of two is exact, and the
+// stopping ratio |d|/|x+| does not depend on it.
 //
 // Persistent scheduling with active-voxel compaction: a wave owns 16 slots;
 // when a slot's voxel converges or reaches max_iter its solution is written
 // out and the slot is refilled with the next voxel from a global counter, so
-// MFMA tiles stay full while iteration counts vary from voxel to voxel.
-// Residual, R^2 and BIC are computed afterwards by a batched epilogue.
+// MFMA tiles stay full while iteration counts vary from voxel to voxel. Empty
+// slots hold zeros and need no predication. Residual, R^2 and BIC are
+// computed afterwards by a batched epilogue.
 // ---------------------------------------------------------------------------
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 
 struct MixedArgs {
     const int64_t* vox;
@@ -422,7 +431,6 @@ struct MixedArgs {
     float thr, step, ratio, alpha_fixed;
     uint32_t max_iter;
     float tol;
-    int nonneg;
     int refresh;
     unsigned int* counter;  // work counter, zero at launch
     float* X;               // [n_vox][Kp] solutions, work order
@@ -443,18 +451,23 @@ __device__ __forceinline__ floatx4 mfma16(uint4 a, uint4 b, floatx4 c) {
 
 template <bool BF16>
 __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-    if constexpr (BF16) {
-        const __bf16 a = (__bf16)lo, b = (__bf16)hi;
-        return uint32_t(__builtin_bit_cast(uint16_t, a)) | (uint32_t(__builtin_bit_cast(uint16_t, b)) << 16);
-    } else {
-        const _Float16 a = (_Float16)lo, b = (_Float16)hi;
-        return uint32_t(__builtin_bit_cast(uint16_t, a)) | (uint32_t(__builtin_bit_cast(uint16_t, b)) << 16);
-    }
+    if constexpr (BF16)
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(float2v{lo, hi}, bf16x2v));
+    else
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(float2v{lo, hi}, half2v));
+}
+
+// 2^(target - E) where bound = m 2^E, m in [0.5, 1); 1 for bound == 0.
+__device__ __forceinline__ float pow2_to(float bound, int target) {
+    if (!(bound > 0.f) || !isfinite(bound)) return 1.f;
+    int e = 0;
+    (void)frexpf(bound, &e);
+    return ldexpf(1.f, target - e);
 }
 
 // B_LDS: keep each slot's b in LDS (needed by the refresh); otherwise in
-// registers (order 10, where FP16 -cG alone takes 128 KB of LDS).
-template <int MB, int WAVES, bool BF16, bool B_LDS>
+// registers (order 10, where the FP16 matrix alone takes 128 KB of LDS).
+template <int MB, int WAVES, bool BF16, bool B_LDS, bool NONNEG>
 __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs a) {
     constexpr int KC = MB / 2;
     constexpr int BSTRIDE = 16 * MB + 4;  // padded slot stride (floats) against LDS bank conflicts
@@ -467,15 +480,15 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     __syncthreads();
 
     floatx4 x[MB], y[MB], acc[MB], breg[MB];
-    int64_t v = -1;
+    int64_t vid = -1;
     uint32_t it = 0;
-    float thr = a.thr, mprev = 0.f, dprev = 0.f;
+    float thr_u = a.thr, thr = a.thr, sc = 1.f, mprev = 0.f, dprev = 0.f, mom = 0.f;
     bool alive = false, zero = false;
 
     // Claims new voxels for the slots flagged in `need` (one atomic per
-    // wave) and loads them: x = y = 0, acc = b. An all-zero voxel of the
-    // alpha-ratio strategy (max|b| = 0) is run like any other; it converges
-    // at its first iteration with x = 0 and the epilogue reports it the way
+    // wave) and loads them: x = y = 0, v = b 2^e. An all-zero voxel of the
+    // alpha-ratio strategy (max|b| = 0) runs like any other: it converges at
+    // its first iteration with x = 0, and is reported the way
     // src/solver/alpha.rs:178-189 does.
     auto refill = [&](bool need) {
         const unsigned long long bal = __ballot(need && h == 0);
@@ -483,41 +496,54 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
         if (lane == 0) base = atomicAdd(a.counter, unsigned(__popcll(bal)));
         base = __shfl(base, 0);
         if (need) {
-            v = int64_t(base) + __popcll(bal & ((1ull << j) - 1ull));
-            alive = v < a.n_vox;
+            vid = int64_t(base) + __popcll(bal & ((1ull << j) - 1ull));
+            alive = vid < a.n_vox;
             it = 0;
-            mprev = dprev = 0.f;
+            mprev = dprev = mom = 0.f;  // mom = (t_0 - 1) / t_1
         }
-        float m = 0.f;
+        float m = 0.f, n2 = 0.f;
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
             floatx4 bq = floatx4{0.f, 0.f, 0.f, 0.f};
-            if (need && alive) bq = *reinterpret_cast<const floatx4*>(a.B + v * a.Kp + 16 * mb + 4 * h);
+            if (need && alive) bq = *reinterpret_cast<const floatx4*>(a.B + vid * a.Kp + 16 * mb + 4 * h);
             if (need) {
-                x[mb] = y[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
-                acc[mb] = bq;
                 if constexpr (B_LDS)
                     *reinterpret_cast<floatx4*>(bsl + 16 * mb) = bq;
                 else
                     breg[mb] = bq;
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) m = fmaxf(m, fabsf(bq[r]));
+            for (int r = 0; r < 4; ++r) {
+                m = fmaxf(m, fabsf(bq[r]));
+                n2 = fmaf(bq[r], bq[r], n2);
+            }
+            if (need) {
+                x[mb] = y[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc[mb] = bq;
+            }
         }
         m = lane_max4(m);  // alpha-ratio: thr_v = ratio * max|b| (src/solver/alpha.rs:229-235)
-        if (need && a.ratio > 0.f) thr = a.ratio * m;
-        if (need) zero = alive && a.ratio > 0.f && m == 0.f;
+        n2 = lane_sum4(n2);
+        if (need) {
+            thr_u = a.ratio > 0.f ? a.ratio * m : a.thr;
+            zero = alive && a.ratio > 0.f && m == 0.f;
+            // the first steps move x by about |b|
+            sc = pow2_to(sqrtf(n2), 10);
+            thr = thr_u * sc;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) acc[mb] *= sc;
+        }
     };
     refill(true);
+
     uint32_t wit = 0;
     const float tol = a.tol;
     for (;;) {
         if (__ballot(alive) == 0ull) break;
-        // An opaque zero offset keeps the (loop-invariant) -cG fragment loads
+        // An opaque zero offset keeps the (loop-invariant) fragment loads
         // inside the loop instead of hoisted into registers for all blocks.
         int moff = 0;
         asm volatile("" : "+s"(moff));
-        const float mom = alive ? a.mom[it] : 0.f;
         float dd = 0.f, xx = 0.f;
         floatx4 dy[MB];
 #pragma unroll
@@ -525,34 +551,36 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const float val = y[mb][r] + acc[mb][r];
-                float xn = val - fminf(fmaxf(val, -thr), thr);
-                if (a.nonneg) xn = fmaxf(xn, 0.f);
+                float xn = val - __builtin_amdgcn_fmed3f(val, -thr, thr);  // strict soft threshold
+                if constexpr (NONNEG) xn = fmaxf(xn, 0.f);
                 const float d = xn - x[mb][r];
                 const float yn = fmaf(mom, d, xn);
                 dd = fmaf(d, d, dd);
                 xx = fmaf(xn, xn, xx);
-                dy[mb][r] = alive ? yn - y[mb][r] : 0.f;
-                if (alive) {
-                    x[mb][r] = xn;
-                    y[mb][r] = yn;
-                }
+                dy[mb][r] = yn - y[mb][r];
+                x[mb][r] = xn;
+                y[mb][r] = yn;
             }
         dd = lane_sum4(dd);
         xx = lane_sum4(xx);
+        const float nd = sqrtf(dd);
         bool done = false, conv = false;
         if (alive) {
             ++it;
-            const float rel = sqrtf(dd) / fmaxf(sqrtf(xx), 1e-12f);
-            conv = rel < tol;
+            conv = nd / fmaxf(sqrtf(xx), 1e-12f * sc) < tol;
             done = conv || it >= a.max_iter;
         }
-        const float nd = sqrtf(dd);
         const float bound = (1.f + mom) * nd + mprev * dprev;
         mprev = mom;
         dprev = nd;
+        // next iteration's momentum, fetched while the MFMA phase runs
+        // (it < max_iter for every slot that is still running)
+        const float mom_next = a.mom[it < a.max_iter ? it : 0];
         ++wit;
-        if (wit % uint32_t(a.refresh) == 0) {
-            // exact refresh: acc = b - c G y on the FP32 MFMA
+        const bool overflow = __ballot(alive && !(bound <= 32768.f)) != 0ull;
+        if (overflow || wit % uint32_t(a.refresh) == 0) {
+            // exact refresh: v = y + (b 2^e - c G y) on the FP32 MFMA, then a
+            // new scale for the next window (capped so |x| stays < 2^40)
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) {
                 floatx4 c;
@@ -560,6 +588,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                     c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
                 else
                     c = breg[mb];
+                c *= sc;
 #pragma unroll
                 for (int kb = 0; kb < MB; ++kb) {
                     const floatx4 mf = a.M32[moff + (mb * MB + kb) * 64 + lane];
@@ -568,35 +597,42 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                 }
                 acc[mb] = c;
             }
-        } else {
-            int e = 0;
-            (void)frexpf(bound, &e);
-            const float sc = bound > 0.f ? ldexpf(1.f, 14 - e) : 1.f;
-            const float isc = bound > 0.f ? ldexpf(1.f, e - 14) : 1.f;
-            uint4 dyh[KC];
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                const floatx4 p = dy[2 * kc] * sc, q = dy[2 * kc + 1] * sc;
-                dyh[kc] = uint4{pack2<BF16>(p[0], p[1]), pack2<BF16>(p[2], p[3]), pack2<BF16>(q[0], q[1]),
-                                pack2<BF16>(q[2], q[3])};
-            }
+            const float f = fminf(pow2_to(bound, 10), pow2_to(sqrtf(xx), 40));
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb) {
-                floatx4 c = floatx4{0.f, 0.f, 0.f, 0.f};
+                x[mb] *= f;
+                y[mb] *= f;
+                acc[mb] *= f;
+            }
+            sc *= f;
+            thr = thr_u * sc;
+            dprev *= f;
+        } else {
+            uint4 dyh[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+                dyh[kc] = uint4{pack2<BF16>(dy[2 * kc][0], dy[2 * kc][1]), pack2<BF16>(dy[2 * kc][2], dy[2 * kc][3]),
+                                pack2<BF16>(dy[2 * kc + 1][0], dy[2 * kc + 1][1]),
+                                pack2<BF16>(dy[2 * kc + 1][2], dy[2 * kc + 1][3])};
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                floatx4 c = acc[mb];
 #pragma unroll
                 for (int kc = 0; kc < KC; ++kc) c = mfma16<BF16>(Ml[moff + (mb * KC + kc) * 64 + lane], dyh[kc], c);
-                acc[mb] += c * isc;
+                acc[mb] = c;
             }
         }
+        mom = mom_next;
         if (__ballot(done) != 0ull) {
             if (done) {
+                const float isc = 1.f / sc;  // exact: sc is a power of two
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb)
-                    *reinterpret_cast<floatx4*>(a.X + v * a.Kp + 16 * mb + 4 * h) = x[mb];
+                    *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
                 if (h == 0) {
-                    a.its[v] = zero ? 0u : it;
-                    a.status[v] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
-                    a.alpha_v[v] = zero ? 0.f : (a.ratio > 0.f ? thr / a.step : a.alpha_fixed);
+                    a.its[vid] = zero ? 0u : it;
+                    a.status[vid] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+                    a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
                 }
             }
             refill(done);
@@ -647,7 +683,6 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.alpha_fixed = float(L.alpha);
     m.max_iter = L.max_iter;
     m.tol = L.tol;
-    m.nonneg = L.nonneg;
     m.refresh = L.refresh < 1 ? 1 : L.refresh;
     m.counter = L.counter;
     m.X = L.Xbuf;
@@ -655,7 +690,7 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.status = L.statusbuf;
     m.alpha_v = L.alphabuf;
     const size_t lds = size_t(MB) * KC * 64 * 16 + (B_LDS ? size_t(WAVES) * 16 * (16 * MB + 4) * 4 : 0);
-    auto kern = fista_mixed_kernel<MB, WAVES, BF16, B_LDS>;
+    auto kern = L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true> : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false>;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(lds));
     int dev = 0, cus = 0, per_cu = 0;
