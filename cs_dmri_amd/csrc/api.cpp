@@ -162,7 +162,7 @@ void gram_and_lipschitz(Design& d, hipStream_t s) {
 struct FistaOps {
     DevBuf<float> M, A;
     DevBuf<double> Ap;  // FP64 design zero-padded to N x Kp
-    DevBuf<uint16_t> Mlow;
+    DevBuf<uint16_t> Mlow, Mlow2;
     double step = 0.0;
 };
 
@@ -178,6 +178,10 @@ void pack_fista(const Design& d, FistaOps& ops, int precision, hipStream_t s) {
     if (precision != SHOREFIT_PREC_FP32) {
         ops.Mlow.alloc(size_t(d.Kp) * d.Kp);
         launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, precision == SHOREFIT_PREC_BF16, ops.Mlow.p, s);
+        if (precision == SHOREFIT_PREC_FP16) {
+            ops.Mlow2.alloc(size_t(d.Kp) * d.Kp);
+            launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, false, ops.Mlow2.p, s, 1);
+        }
     }
 }
 
@@ -199,6 +203,7 @@ void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& o
     L.Bbuf = w.B.p;
     L.meanbuf = w.mean.p;
     L.Mlow = ops.Mlow.p;
+    L.Mlow2 = ops.Mlow2.p;
     if (L.precision == SHOREFIT_PREC_FP32) return;
     // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112), in FP64
     std::vector<float> mom(std::max<uint32_t>(L.max_iter, 1));

@@ -74,8 +74,10 @@ void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s);
 void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hipStream_t s);
 void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, bool transpose, double scale,
                           int rows_p, int cols_p, float* d_frag, hipStream_t s);
-// -c G in FP16 / BF16 16x16x32 fragment order (mixed kernel increments).
-void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s);
+// -c G in FP16 / BF16 16x16x32 fragment order (mixed kernel increments);
+// part 1 packs the FP16 rounding remainder scaled by 2^12 (split refresh).
+void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s,
+                        int part = 0);
 // dst (rows x cols_p, FP64) = src (rows x cols) zero-padded on the right.
 void launch_pad_cols(const double* d_src, int rows, int cols, int cols_p, double* d_dst, hipStream_t s);
 
@@ -93,6 +95,7 @@ struct FistaLaunch {
     const float* ATfrag;
     const float* Afrag;
     const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
+    const uint16_t* Mlow2;  // FP16 remainder of -cG x 2^12 (split refresh, FP16 only)
     const double* A64p;     // FP64 design zero-padded to N x Kp (right-hand sides)
     double c;               // step / N
     float* Bbuf;            // scratch [n_vox][Kp]: b = c A^T s

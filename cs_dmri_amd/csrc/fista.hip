@@ -400,8 +400,10 @@ void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
 // rounding is relative to |dy|, which shrinks as the iterate converges (acc,
 // not the larger y + acc, carries the FP32 accumulation, which keeps the
 // rounding noise under the stopping test small). Every `refresh` iterations
-// acc is recomputed exactly as b 2^e - c G y on the FP32 MFMA (FP32 -cG read
-// from L2), which bounds the drift, and the scale is re-chosen: 2^e puts the
+// acc is recomputed exactly as b 2^e - c G y, which bounds the drift: with
+// FP16 increments from hi/lo FP16 splits of -cG and y on three FP16 MFMA
+// passes (~2^-22 relative, FP32-grade); with BF16 increments on the FP32
+// MFMA (FP32 -cG read from L2). The scale is then re-chosen: 2^e puts the
 // bound |dy| <= (1 + m_k)|d_k| + m_{k-1}|d_{k-1}| near 2^10, so the FP16 copy
 // of dy has headroom; a slot whose bound would pass 2^15 forces an exact
 // refresh before it is converted. Scaling by powers of two is exact, and the
@@ -427,6 +429,7 @@ struct MixedArgs {
     const float* B;         // [n_vox][Kp]
     const floatx4* M32;     // -cG, 16x16x4 f32 fragments (MB*MB*64)
     const uint4* M16;       // -cG, 16x16x32 f16/bf16 fragments (MB*KC*64)
+    const uint4* M16lo;     // FP16 remainder of -cG, x 2^12 (split refresh)
     const float* mom;       // momentum (t_k - 1)/t_{k+1}, k < max_iter
     float thr, step, ratio, alpha_fixed;
     uint32_t max_iter;
@@ -467,7 +470,9 @@ __device__ __forceinline__ float pow2_to(float bound, int target) {
 
 // B_LDS: keep each slot's b in LDS (needed by the refresh); otherwise in
 // registers (order 10, where the FP16 matrix alone takes 128 KB of LDS).
-template <int MB, int WAVES, bool BF16, bool B_LDS, bool NONNEG>
+// SPLIT: refresh with FP16 hi/lo splits (three FP16 MFMA passes) instead of
+// the FP32 MFMA (FP16 increments only).
+template <int MB, int WAVES, bool BF16, bool B_LDS, bool NONNEG, bool SPLIT>
 __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs a) {
     constexpr int KC = MB / 2;
     constexpr int BSTRIDE = 16 * MB + 4;  // padded slot stride (floats) against LDS bank conflicts
@@ -579,23 +584,72 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
         ++wit;
         const bool overflow = __ballot(alive && !(bound <= 32768.f)) != 0ull;
         if (overflow || wit % uint32_t(a.refresh) == 0) {
-            // exact refresh: v = y + (b 2^e - c G y) on the FP32 MFMA, then a
-            // new scale for the next window (capped so |x| stays < 2^40)
+            // exact refresh: acc = b 2^e - c G y, then a new scale for the
+            // next window (capped so |x| stays < 2^40)
+            if constexpr (SPLIT) {
+                // -cG y from FP16 hi/lo splits of both operands:
+                // hi*hi + hi*lo on one chain, lo*hi (lo x 2^12) on another;
+                // the dropped lo*lo term is ~2^-22 of |cG||y|, like FP32.
+                float ym = 0.f;
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) {
-                floatx4 c;
-                if constexpr (B_LDS)
-                    c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
-                else
-                    c = breg[mb];
-                c *= sc;
+                for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                for (int kb = 0; kb < MB; ++kb) {
-                    const floatx4 mf = a.M32[moff + (mb * MB + kb) * 64 + lane];
+                    for (int r = 0; r < 4; ++r) ym = fmaxf(ym, fabsf(y[mb][r]));
+                ym = lane_max4(ym);
+                const float sy = pow2_to(ym, 14), isy = 1.f / sy;
+                uint4 yh[KC], yl[KC];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+                for (int kc = 0; kc < KC; ++kc) {
+                    float t[8], e[8];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        t[q] = y[2 * kc][q] * sy;
+                        t[4 + q] = y[2 * kc + 1][q] * sy;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) e[q] = t[q] - float(_Float16(t[q]));
+                    yh[kc] = uint4{pack2<false>(t[0], t[1]), pack2<false>(t[2], t[3]), pack2<false>(t[4], t[5]),
+                                   pack2<false>(t[6], t[7])};
+                    yl[kc] = uint4{pack2<false>(e[0], e[1]), pack2<false>(e[2], e[3]), pack2<false>(e[4], e[5]),
+                                   pack2<false>(e[6], e[7])};
                 }
-                acc[mb] = c;
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    floatx4 c1 = floatx4{0.f, 0.f, 0.f, 0.f}, c2 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc) {
+                        const uint4 mh = Ml[moff + (mb * KC + kc) * 64 + lane];
+                        const uint4 ml = a.M16lo[moff + (mb * KC + kc) * 64 + lane];
+                        c1 = mfma16<false>(mh, yh[kc], c1);
+                        c1 = mfma16<false>(mh, yl[kc], c1);
+                        c2 = mfma16<false>(ml, yh[kc], c2);
+                    }
+                    floatx4 bq;
+                    if constexpr (B_LDS)
+                        bq = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
+                    else
+                        bq = breg[mb];
+                    acc[mb] = bq * sc + (c1 + c2 * (1.f / 4096.f)) * isy;
+                    // one output block at a time: bounds the loads in flight
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            } else {
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    floatx4 c;
+                    if constexpr (B_LDS)
+                        c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
+                    else
+                        c = breg[mb];
+                    c *= sc;
+#pragma unroll
+                    for (int kb = 0; kb < MB; ++kb) {
+                        const floatx4 mf = a.M32[moff + (mb * MB + kb) * 64 + lane];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+                    }
+                    acc[mb] = c;
+                }
             }
             const float f = fminf(pow2_to(bound, 10), pow2_to(sqrtf(xx), 40));
 #pragma unroll
@@ -676,6 +730,7 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.B = L.Bbuf;
     m.M32 = reinterpret_cast<const floatx4*>(L.Mfrag);
     m.M16 = reinterpret_cast<const uint4*>(L.Mlow);
+    m.M16lo = reinterpret_cast<const uint4*>(L.Mlow2);
     m.mom = L.mom;
     m.thr = L.thr;
     m.step = L.step;
@@ -690,7 +745,9 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.status = L.statusbuf;
     m.alpha_v = L.alphabuf;
     const size_t lds = size_t(MB) * KC * 64 * 16 + (B_LDS ? size_t(WAVES) * 16 * (16 * MB + 4) * 4 : 0);
-    auto kern = L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true> : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false>;
+    constexpr bool SPLIT = !BF16;
+    auto kern = L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true, SPLIT>
+                         : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false, SPLIT>;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(lds));
     int dev = 0, cus = 0, per_cu = 0;

@@ -307,7 +307,9 @@ __global__ void pack_M_kernel(const double* __restrict__ G, int K, int Kp, doubl
 // lane l = 16h + i, eight values M[16mb + i][p(v)], v = 0..7, where
 // p(v) = 32kc + 16(v/4) + 4h + v%4 is the row of Y that lane (h, j) of the
 // accumulator layout holds in register v%4 of block 2kc + v/4.
-__global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, double c, int bf16,
+// part 0: the FP16 (BF16) rounding of -cG; part 1 (FP16 only): the rounding
+// remainder -cG - fp16(-cG), scaled by 2^12 so it stays in FP16's normal range.
+__global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, double c, int bf16, int part,
                                    uint16_t* __restrict__ frag) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int KC = Kp / 32;
@@ -324,6 +326,7 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
         frag[t] = *reinterpret_cast<uint16_t*>(&b);
     } else {
         __half hv = __float2half(float(x));
+        if (part == 1) hv = __float2half(float((x - double(__half2float(hv))) * 4096.0));
         frag[t] = *reinterpret_cast<uint16_t*>(&hv);
     }
 }
@@ -409,10 +412,11 @@ void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, b
     SF_HIP(hipGetLastError());
 }
 
-void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s) {
+void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s,
+                        int part) {
     const int64_t total = int64_t(Kp) * Kp;
     hipLaunchKernelGGL(pack_M_lowp_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_G, K, Kp, c,
-                       int(bf16), d_frag);
+                       int(bf16), part, d_frag);
     SF_HIP(hipGetLastError());
 }
 
