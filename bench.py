@@ -41,7 +41,22 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=4096)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--max-iter", type=int, default=500)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB (three launches) and exit: the known-size "
+                        "reference for scaling HBM byte counters")
     return p.parse_args()
+
+
+def copy_calibration(mib: int):
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches": 3}}), flush=True)
 
 
 def main():
@@ -50,6 +65,9 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not supported "
               "(the voxel loop would shard by voxel ranges with no communication).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import torch
 

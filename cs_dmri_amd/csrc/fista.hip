@@ -29,15 +29,25 @@ __device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float lane_sum4(float v) {
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
+// Combine lanes l, l^16, l^32, l^48 (the four 4-row groups of one voxel
+// column in the MFMA D layout) with the gfx950 lane swaps: permlane32_swap of
+// a register with itself leaves lanes 0-31 of the pair in the first result
+// and lanes 32-63 in the second, so their sum holds x[l] + x[l^32] in every
+// lane; permlane16_swap does the same for rows of 16. VALU only, no LDS.
+__device__ __forceinline__ float xor32_sum(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
+__device__ __forceinline__ float xor16_sum(float v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float lane_sum4(float v) { return xor32_sum(xor16_sum(v)); }
 __device__ __forceinline__ float lane_max4(float v) {
-    v = fmaxf(v, __shfl_xor(v, 16));
-    v = fmaxf(v, __shfl_xor(v, 32));
-    return v;
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+    r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
 struct FistaArgs {
@@ -488,6 +498,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     int64_t vid = -1;
     uint32_t it = 0;
     float thr_u = a.thr, thr = a.thr, sc = 1.f, mprev = 0.f, dprev = 0.f, mom = 0.f;
+    float xref = 0.f, drift = 0.f;  // |x| at its last evaluation, sum of |dx| since
     bool alive = false, zero = false;
 
     // Claims new voxels for the slots flagged in `need` (one atomic per
@@ -505,6 +516,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
             alive = vid < a.n_vox;
             it = 0;
             mprev = dprev = mom = 0.f;  // mom = (t_0 - 1) / t_1
+            xref = drift = 0.f;
         }
         float m = 0.f, n2 = 0.f;
 #pragma unroll
@@ -541,156 +553,239 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     };
     refill(true);
 
-    uint32_t wit = 0;
-    const float tol = a.tol;
-    for (;;) {
-        if (__ballot(alive) == 0ull) break;
-        // An opaque zero offset keeps the (loop-invariant) fragment loads
-        // inside the loop instead of hoisted into registers for all blocks.
-        int moff = 0;
-        asm volatile("" : "+s"(moff));
-        float dd = 0.f, xx = 0.f;
-        floatx4 dy[MB];
+    // |x| of every slot's voxel (FP32 sum of squares in a fixed order).
+    // The opaque start value keeps the compiler from merging the two uses
+    // and hoisting the sum onto the every-iteration path.
+    auto xnorm = [&]() {
+        float q = 0.f;
+        asm volatile("" : "+v"(q));
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float val = y[mb][r] + acc[mb][r];
-                float xn = val - __builtin_amdgcn_fmed3f(val, -thr, thr);  // strict soft threshold
-                if constexpr (NONNEG) xn = fmaxf(xn, 0.f);
-                const float d = xn - x[mb][r];
-                const float yn = fmaf(mom, d, xn);
-                dd = fmaf(d, d, dd);
-                xx = fmaf(xn, xn, xx);
-                dy[mb][r] = yn - y[mb][r];
-                x[mb][r] = xn;
-                y[mb][r] = yn;
-            }
-        dd = lane_sum4(dd);
-        xx = lane_sum4(xx);
-        const float nd = sqrtf(dd);
+            for (int r = 0; r < 4; ++r) q = fmaf(x[mb][r], x[mb][r], q);
+        return sqrtf(lane_sum4(q));
+    };
+
+    uint32_t wit = 0;
+    const float tol = a.tol;
+    // Outer loop: write out finished slots and refill them. Inner loop:
+    // iterate until some slot finishes. Keeping the refill (which rewrites
+    // x, y, acc) out of the inner loop keeps register copies off the
+    // per-iteration path.
+    for (;;) {
+        if (__ballot(alive) == 0ull) break;
         bool done = false, conv = false;
-        if (alive) {
-            ++it;
-            conv = nd / fmaxf(sqrtf(xx), 1e-12f * sc) < tol;
-            done = conv || it >= a.max_iter;
-        }
-        const float bound = (1.f + mom) * nd + mprev * dprev;
-        mprev = mom;
-        dprev = nd;
-        // next iteration's momentum, fetched while the MFMA phase runs
-        // (it < max_iter for every slot that is still running)
-        const float mom_next = a.mom[it < a.max_iter ? it : 0];
-        ++wit;
-        const bool overflow = __ballot(alive && !(bound <= 32768.f)) != 0ull;
-        if (overflow || wit % uint32_t(a.refresh) == 0) {
-            // exact refresh: acc = b 2^e - c G y, then a new scale for the
-            // next window (capped so |x| stays < 2^40)
-            if constexpr (SPLIT) {
-                // -cG y from FP16 hi/lo splits of both operands:
-                // hi*hi + hi*lo on one chain, lo*hi (lo x 2^12) on another;
-                // the dropped lo*lo term is ~2^-22 of |cG||y|, like FP32.
-                float ym = 0.f;
+        for (;;) {
+            // An opaque zero offset keeps the (loop-invariant) fragment loads
+            // inside the loop instead of hoisted into registers for all blocks.
+            int moff = 0;
+            asm volatile("" : "+s"(moff));
+            float dd = 0.f;
+            // dy = y_new - y_old goes to FP16/BF16 here, while y_old is still
+            // in its registers; the opaque asm keeps the conversion from
+            // being sunk into the increment branch, which would keep both
+            // copies of y alive across the branch.
+            uint32_t dyh[KC][4];
 #pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
+            for (int kc = 0; kc < KC; ++kc) {
+                float t[8];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) ym = fmaxf(ym, fabsf(y[mb][r]));
-                ym = lane_max4(ym);
-                const float sy = pow2_to(ym, 14), isy = 1.f / sy;
-                uint4 yh[KC], yl[KC];
+                for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-                for (int kc = 0; kc < KC; ++kc) {
-                    float t[8], e[8];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        t[q] = y[2 * kc][q] * sy;
-                        t[4 + q] = y[2 * kc + 1][q] * sy;
+                    for (int r = 0; r < 4; ++r) {
+                        const int mb = 2 * kc + s2;
+                        const float val = y[mb][r] + acc[mb][r];
+                        float xn = val - __builtin_amdgcn_fmed3f(val, -thr, thr);  // strict soft threshold
+                        if constexpr (NONNEG) xn = fmaxf(xn, 0.f);
+                        const float d = xn - x[mb][r];
+                        const float yn = fmaf(mom, d, xn);
+                        dd = fmaf(d, d, dd);
+                        t[4 * s2 + r] = yn - y[mb][r];
+                        x[mb][r] = xn;
+                        y[mb][r] = yn;
                     }
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) e[q] = t[q] - float(_Float16(t[q]));
-                    yh[kc] = uint4{pack2<false>(t[0], t[1]), pack2<false>(t[2], t[3]), pack2<false>(t[4], t[5]),
-                                   pack2<false>(t[6], t[7])};
-                    yl[kc] = uint4{pack2<false>(e[0], e[1]), pack2<false>(e[2], e[3]), pack2<false>(e[4], e[5]),
-                                   pack2<false>(e[6], e[7])};
+                for (int q = 0; q < 4; ++q) {
+                    dyh[kc][q] = pack2<BF16>(t[2 * q], t[2 * q + 1]);
+                    asm volatile("" : "+v"(dyh[kc][q]));
                 }
+            }
+            dd = lane_sum4(dd);
+            const float nd = __builtin_amdgcn_sqrtf(dd);  // V_SQRT_F32, 1 ulp
+            // Stopping rule nd / max(|x_new|, 1e-12) < tol without summing
+            // |x_new|^2 every iteration: |x_new| lies within xref -+ drift
+            // (triangle inequality over the steps since xref was computed),
+            // widened by 4e-5 for the rounding of the FP32 norms. Only when
+            // that interval straddles the threshold is |x_new| summed, in the
+            // same order as always, so the decision is the one the exact test
+            // gives.
+            drift += nd;
+            const float fl = 1e-12f * sc;
+            const float lo = fmaxf(fmaf(xref, 1.f - 4e-5f, -drift * (1.f + 4e-5f)), fl);
+            const float hi = fmaxf((xref + drift) * (1.f + 4e-5f), fl);
+            bool cv = nd < tol * lo * (1.f - 1e-6f);
+            const bool amb = !cv && !(nd > tol * hi * (1.f + 1e-6f));
+            if (__ballot(alive && amb) != 0ull) {
+                const float xs = xnorm();
+                if (amb) cv = nd / fmaxf(xs, fl) < tol;
+                xref = xs;
+                drift = 0.f;
+            }
+            if (alive) {
+                ++it;
+                conv = cv;
+                done = conv || it >= a.max_iter;
+            }
+            const float bound = (1.f + mom) * nd + mprev * dprev;
+            mprev = mom;
+            dprev = nd;
+            // next iteration's momentum, fetched while the MFMA phase runs
+            // (it < max_iter for every slot that is still running)
+            const float mom_next = a.mom[it < a.max_iter ? it : 0];
+            ++wit;
+            const bool overflow = __ballot(alive && !(bound <= 32768.f)) != 0ull;
+            if (overflow || wit % uint32_t(a.refresh) == 0) {
+                // exact refresh: a new scale 2^e for the next window (capped
+                // so |x| stays < 2^40), then acc = b 2^e - c G y.
+                // Matrix fragments from global memory (L2): a uniform base plus
+                // a 32-bit per-lane offset (one VALU add per load).
+                const uint32_t loff = (uint32_t(moff) + uint32_t(lane)) * 16u;
+                const char* lo16 = reinterpret_cast<const char*>(a.M16lo);
+                constexpr int LA = MB < 2 ? MB : 2;  // blocks of remainder fragments in flight
+                uint4 glo[LA + 1][KC];
+                if constexpr (SPLIT) {
+                    // the first blocks' remainder fragments are requested before
+                    // the VALU work below, which covers their L2 latency
 #pragma unroll
-                for (int mb = 0; mb < MB; ++mb) {
-                    floatx4 c1 = floatx4{0.f, 0.f, 0.f, 0.f}, c2 = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int kc = 0; kc < KC; ++kc) {
-                        const uint4 mh = Ml[moff + (mb * KC + kc) * 64 + lane];
-                        const uint4 ml = a.M16lo[moff + (mb * KC + kc) * 64 + lane];
-                        c1 = mfma16<false>(mh, yh[kc], c1);
-                        c1 = mfma16<false>(mh, yl[kc], c1);
-                        c2 = mfma16<false>(ml, yh[kc], c2);
-                    }
-                    floatx4 bq;
-                    if constexpr (B_LDS)
-                        bq = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
-                    else
-                        bq = breg[mb];
-                    acc[mb] = bq * sc + (c1 + c2 * (1.f / 4096.f)) * isy;
-                    // one output block at a time: bounds the loads in flight
+                    for (int i = 0; i < LA * KC; ++i)
+                        glo[i / KC][i % KC] = *reinterpret_cast<const uint4*>(lo16 + (loff + uint32_t(i) * 1024u));
                     __builtin_amdgcn_sched_barrier(0);
                 }
-            } else {
+                const float xs = xnorm();
+                const float f = fminf(pow2_to(bound, 10), pow2_to(xs, 40));
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
-                    floatx4 c;
-                    if constexpr (B_LDS)
-                        c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
-                    else
-                        c = breg[mb];
-                    c *= sc;
+                    x[mb] *= f;
+                    y[mb] *= f;
+                }
+                sc *= f;
+                thr = thr_u * sc;
+                dprev *= f;
+                xref = xs * f;
+                drift = 0.f;
+                if constexpr (SPLIT) {
+                    // -cG y from FP16 hi/lo splits of both operands:
+                    // hi*hi + hi*lo on one chain, lo*hi (lo x 2^12) on another;
+                    // the dropped lo*lo term is ~2^-22 of |cG||y|, like FP32.
+                    float ym = 0.f;
 #pragma unroll
-                    for (int kb = 0; kb < MB; ++kb) {
-                        const floatx4 mf = a.M32[moff + (mb * MB + kb) * 64 + lane];
+                    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+                        for (int r = 0; r < 4; ++r) ym = fmaxf(ym, fabsf(y[mb][r]));
+                    ym = lane_max4(ym);
+                    const float sy = pow2_to(ym, 14), isy = 1.f / sy;
+                    uint4 yh[KC], yl[KC];
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc) {
+                        float t[8], e[8];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            t[q] = y[2 * kc][q] * sy;
+                            t[4 + q] = y[2 * kc + 1][q] * sy;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) e[q] = t[q] - float(_Float16(t[q]));
+                        yh[kc] = uint4{pack2<false>(t[0], t[1]), pack2<false>(t[2], t[3]), pack2<false>(t[4], t[5]),
+                                       pack2<false>(t[6], t[7])};
+                        yl[kc] = uint4{pack2<false>(e[0], e[1]), pack2<false>(e[2], e[3]), pack2<false>(e[4], e[5]),
+                                       pack2<false>(e[6], e[7])};
                     }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb) {
+                        // later blocks' remainder fragments load while this
+                        // block's MFMAs run
+                        if (mb + LA < MB) {
+#pragma unroll
+                            for (int kc = 0; kc < KC; ++kc)
+                                glo[(mb + LA) % (LA + 1)][kc] = *reinterpret_cast<const uint4*>(
+                                    lo16 + (loff + uint32_t((mb + LA) * KC + kc) * 1024u));
+                        }
+                        floatx4 c1 = floatx4{0.f, 0.f, 0.f, 0.f}, c2 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int kc = 0; kc < KC; ++kc) {
+                            const uint4 mh = Ml[moff + (mb * KC + kc) * 64 + lane];
+                            c1 = mfma16<false>(mh, yh[kc], c1);
+                            c1 = mfma16<false>(mh, yl[kc], c1);
+                            c2 = mfma16<false>(glo[mb % (LA + 1)][kc], yh[kc], c2);
+                        }
+                        floatx4 bq;
+                        if constexpr (B_LDS)
+                            bq = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
+                        else
+                            bq = breg[mb];
+                        acc[mb] = bq * sc + (c1 + c2 * (1.f / 4096.f)) * isy;
+                        // one output block at a time: bounds the registers in flight
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                } else {
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb) {
+                        floatx4 c;
+                        if constexpr (B_LDS)
+                            c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
+                        else
+                            c = breg[mb];
+                        c *= sc;
+#pragma unroll
+                        for (int kb = 0; kb < MB; ++kb) {
+                            const floatx4 mf = *reinterpret_cast<const floatx4*>(
+                                reinterpret_cast<const char*>(a.M32) + (loff + uint32_t(mb * MB + kb) * 1024u));
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+                        }
+                        acc[mb] = c;
+                    }
+                }
+            } else {
+                // acc += (-cG) fp16(dy). The schedule keeps the fragments of
+                // the next two output blocks in flight from LDS while one
+                // block's MFMAs run (left to itself the scheduler keeps two
+                // fragments in flight and the MFMAs wait on LDS latency).
+                uint4 fr[MB * KC];
+#pragma unroll
+                for (int i = 0; i < MB * KC; ++i) fr[i] = Ml[moff + i * 64 + lane];
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    floatx4 c = acc[mb];
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc)
+                        c = mfma16<BF16>(fr[mb * KC + kc], uint4{dyh[kc][0], dyh[kc][1], dyh[kc][2], dyh[kc][3]}, c);
                     acc[mb] = c;
                 }
-            }
-            const float f = fminf(pow2_to(bound, 10), pow2_to(sqrtf(xx), 40));
+                constexpr int AHEAD = MB < 2 ? MB : 2;
+                __builtin_amdgcn_sched_group_barrier(0x100, AHEAD * KC, 0);  // DS reads
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) {
-                x[mb] *= f;
-                y[mb] *= f;
-                acc[mb] *= f;
-            }
-            sc *= f;
-            thr = thr_u * sc;
-            dprev *= f;
-        } else {
-            uint4 dyh[KC];
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc)
-                dyh[kc] = uint4{pack2<BF16>(dy[2 * kc][0], dy[2 * kc][1]), pack2<BF16>(dy[2 * kc][2], dy[2 * kc][3]),
-                                pack2<BF16>(dy[2 * kc + 1][0], dy[2 * kc + 1][1]),
-                                pack2<BF16>(dy[2 * kc + 1][2], dy[2 * kc + 1][3])};
-#pragma unroll
-            for (int mb = 0; mb < MB; ++mb) {
-                floatx4 c = acc[mb];
-#pragma unroll
-                for (int kc = 0; kc < KC; ++kc) c = mfma16<BF16>(Ml[moff + (mb * KC + kc) * 64 + lane], dyh[kc], c);
-                acc[mb] = c;
-            }
-        }
-        mom = mom_next;
-        if (__ballot(done) != 0ull) {
-            if (done) {
-                const float isc = 1.f / sc;  // exact: sc is a power of two
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-                    *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
-                if (h == 0) {
-                    a.its[vid] = zero ? 0u : it;
-                    a.status[vid] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
-                    a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
+                for (int mb = 0; mb < MB; ++mb) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, KC, 0);  // MFMA
+                    if (mb + AHEAD < MB) __builtin_amdgcn_sched_group_barrier(0x100, KC, 0);
                 }
             }
-            refill(done);
+            mom = mom_next;
+            if (__ballot(done) != 0ull) break;
         }
+        if (done) {
+            const float isc = 1.f / sc;  // exact: sc is a power of two
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+                *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
+            if (h == 0) {
+                a.its[vid] = zero ? 0u : it;
+                a.status[vid] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+                a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
+            }
+        }
+        refill(done);
     }
 }
 
