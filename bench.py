@@ -29,6 +29,27 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "voxel fits/s (SHORE-8 FISTA, coef err≤1e-4 vs FP64) + % MFMA/HBM roofline"
 PEAK = {"fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
+KERNEL = {"fp32": "fista_f32_kernel<10,8,true>", "fp16": "fista_mixed_kernel<10,8,false,true,false,true>",
+          "bf16": "fista_mixed_kernel<10,8,true,true,false,false>"}
+# HBM bytes per launch of the solve kernel on this workload, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch), each scaled by
+# what the counter reads for a plain 1 GiB device copy (bench.py
+# --copy-calibration 1024); the summaries under profiles/ hold the commands.
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
+
+def traffic_bytes(precision: str):
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())[precision]
+    except (OSError, KeyError, ValueError):
+        return None
+    return round(t["fetch_kib"] * t["fetch_scale"] * 1024 + t["write_kib"] * t["write_scale"] * 1024)
+
+
+def d_Kp(K: int, precision: str) -> int:
+    """Padded coefficient count the kernels use (multiples of 16 / 32)."""
+    sizes = [16, 32, 48, 64, 80, 96, 128, 160, 192, 256] if precision == "fp32" else [32, 64, 96, 128, 160, 256]
+    return next(s for s in sizes if s >= K)
 
 
 def parse():
@@ -38,9 +59,11 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
     p.add_argument("--refresh", type=int, default=10)
-    p.add_argument("--cpu-sample", type=int, default=4096)
+    p.add_argument("--cpu-sample", type=int, default=32768)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--max-iter", type=int, default=500)
+    p.add_argument("--alpha-mode", default="fixed", choices=["fixed", "alpha-ratio", "path-bic", "l2-anchored"],
+                   help="alpha strategy (the headline line is the fixed-alpha fit)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (three launches) and exit: the known-size "
                         "reference for scaling HBM byte counters")
@@ -90,8 +113,8 @@ def main():
     torch.cuda.synchronize()
 
     from cs_dmri_amd.models import _spec
-    spec = _spec(wl.radial_order, wl.zeta, "l1", "fixed", 1e-3, 1e-3, 20, None, 0.05, args.max_iter, 1e-6, False,
-                 1e-8, 1e-8, True, False, args.precision, args.refresh)
+    spec = _spec(wl.radial_order, wl.zeta, "l1", args.alpha_mode, 1e-3, 1e-3, 20, None, 0.05, args.max_iter, 1e-6,
+                 False, 1e-8, 1e-8, True, False, args.precision, args.refresh)
     grads, keep = L.gradients(g.bvals, g.bvecs, g.big_delta, g.small_delta, g.b0_threshold)
     vol = L.ShorefitVolume(d_data.data_ptr(), d_mask.data_ptr(), shape[0], shape[1], shape[2], 1, 0)
 
@@ -116,17 +139,34 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     value = V * args.steps / wall
 
-    # --- roofline of the dominant kernel (FISTA) -----------------------------
+    # --- roofline of the dominant kernel (the FISTA solve kernel) -------------
+    # Algorithmic work per voxel-iteration is the reference algorithm's
+    # (src/solver/fista.rs:89-107, residual form): A y and A^T r, 4 N K
+    # MAC-flops, plus N + 12 K element-wise flops. The kernel executes the
+    # equivalent Gram form (2 Kp^2 per increment, 3 x 2 Kp^2 per split-FP16
+    # refresh, FP32 MFMA for fp32), reported beside it as "executed".
     its = d_iter.cpu().numpy()[wl.mask].astype(np.int64)
-    # Gram form: 2 K^2 MAC-flops per voxel-iteration (M y) + 2 N K (A^T s) + 2 N K (A x).
-    flops = float(its.sum()) * 2 * K * K + V * 4.0 * N * K
+    its_sum = float(its.sum())
+    per_it = 4.0 * N * K + N + 12.0 * K
+    flops = its_sum * per_it
     kern_s = float(np.mean(kernel_ms)) * 1e-3
     achieved = flops / kern_s / 1e12
     peak = PEAK[args.precision]
+    Kp = d_Kp(K, args.precision)
+    if args.precision == "fp32":
+        executed = its_sum * 2.0 * Kp * Kp
+    else:
+        # increments on FP16/BF16 MFMA; refresh every k iterations: three FP16
+        # passes (fp16) or one FP32 MFMA pass (bf16)
+        k = args.refresh
+        executed = its_sum * 2.0 * Kp * Kp * ((k - 1) / k + (3.0 if args.precision == "fp16" else 1.0) / k)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
-                "kernel": f"fista_{args.precision}_kernel", "kernel_ms": round(kern_s * 1e3, 3),
-                "flops_per_launch": flops}
+                "frac": round(achieved / peak, 4), "traffic": traffic_bytes(args.precision),
+                "kernel": KERNEL[args.precision], "kernel_ms": round(kern_s * 1e3, 3),
+                "flops_per_launch": flops, "flops_per_voxel_iteration": per_it,
+                "executed": {"mfma_flops_per_launch": executed,
+                             "tflops": round(executed / kern_s / 1e12, 3),
+                             "frac": round(executed / kern_s / 1e12 / peak, 4)}}
 
     coef = d_coef.cpu().numpy()[wl.mask]
     result = {
@@ -135,7 +175,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic (seeded multi-tensor + Rician phantom)",
         "config": {"workload": "shore8-cartesian258-96x96x60-ellipsoid", "voxels": V, "n_grads": N,
-                   "n_coeffs": K, "radial_order": 8, "solver": "fista-l1 fixed alpha=1e-3",
+                   "n_coeffs": K, "radial_order": 8, "solver": ("fista-l1 fixed alpha=1e-3" if args.alpha_mode == "fixed"
+                                                             else f"fista-l1 {args.alpha_mode}"),
                    "max_iter": args.max_iter, "tol": 1e-6, "precision": args.precision,
                    "refresh_every": args.refresh if args.precision != "fp32" else None},
         "iterations": {"mean": float(its.mean()), "p50": float(np.median(its)), "p99": float(np.quantile(its, 0.99)),
@@ -144,7 +185,7 @@ def main():
     }
 
     # --- CPU baseline leg: the FP64 oracle on a bounded sample ------------------
-    if not args.no_cpu:
+    if not args.no_cpu and args.alpha_mode == "fixed":
         from oracle import load
         orc = load()
         A = orc.design_matrix(wl.radial_order, wl.zeta, g.bvals, g.bvecs, g.big_delta, g.small_delta)

@@ -189,10 +189,10 @@ void pack_fista(const Design& d, FistaOps& ops, int precision, hipStream_t s) {
 // mixed-precision persistent kernel its momentum table, work counter and
 // work-order results.
 struct FistaWork {
-    DevBuf<float> B, mean, mom, X, alpha;
+    DevBuf<float> B, mean, mom, X, alpha, Xpath;
     DevBuf<unsigned int> counter;
-    DevBuf<uint32_t> its;
-    DevBuf<uint8_t> status;
+    DevBuf<uint32_t> its, itspath;
+    DevBuf<uint8_t> status, convpath;
 };
 
 void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& ops, int64_t V, hipStream_t s) {
@@ -204,7 +204,7 @@ void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& o
     L.meanbuf = w.mean.p;
     L.Mlow = ops.Mlow.p;
     L.Mlow2 = ops.Mlow2.p;
-    if (L.precision == SHOREFIT_PREC_FP32) return;
+    if (L.precision == SHOREFIT_PREC_FP32 && L.path_n == 0) return;
     // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112), in FP64
     std::vector<float> mom(std::max<uint32_t>(L.max_iter, 1));
     double t = 1.0;
@@ -215,12 +215,22 @@ void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& o
     }
     w.mom.alloc(mom.size());
     SF_HIP(hipMemcpyAsync(w.mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
+    L.mom = w.mom.p;
+    if (L.path_n > 0) {
+        w.Xpath.alloc(size_t(V) * L.path_n * d.Kp);
+        w.itspath.alloc(size_t(V) * L.path_n);
+        w.convpath.alloc(size_t(V) * L.path_n);
+        L.Xpath = w.Xpath.p;
+        L.itspath = w.itspath.p;
+        L.convpath = w.convpath.p;
+    }
+    SF_HIP(hipStreamSynchronize(s));  // the host momentum table goes out of scope
+    if (L.precision == SHOREFIT_PREC_FP32) return;
     w.counter.alloc(1);
     w.X.alloc(size_t(V) * d.Kp);
     w.its.alloc(V);
     w.status.alloc(V);
     w.alpha.alloc(V);
-    L.mom = w.mom.p;
     L.counter = w.counter.p;
     L.Xbuf = w.X.p;
     L.itsbuf = w.its.p;
@@ -543,11 +553,6 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                                 " coefficients (radial order " + std::to_string(sp.radial_order) +
                                 "): without L1 regularization the fit is underdetermined and its fit statistics "
                                 "are not meaningful; use a lower radial order or L1");
-        if (sp.regularization == SHOREFIT_REG_L1 &&
-            (sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC || sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED))
-            throw Error{SHOREFIT_E_UNSUPPORTED,
-                        "alpha path strategies (path-bic, l2-anchored) are not implemented on this device yet; "
-                        "use alpha_mode 'fixed' or 'alpha-ratio'"};
 
         // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
         const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
@@ -595,6 +600,14 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         uint8_t* kind = out_buf(out->regularization_kind, d_kind, nvox_total);
         float* amap = l1 ? out_buf(out->alpha, d_alpha, nvox_total) : nullptr;
         float* bic = l1 ? out_buf(out->bic, d_bic, nvox_total) : nullptr;
+        const bool path = l1 && (sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC || sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED);
+        const bool anchored = l1 && sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED;
+        // L2-anchored: the Tikhonov reference RSS per voxel (also the rss_l2 map)
+        float* rssl2 = anchored ? out_buf(out->rss_l2, d_rssl2, nvox_total) : nullptr;
+        if (anchored && !rssl2) {
+            d_rssl2.alloc(nvox_total);
+            rssl2 = d_rssl2.p;
+        }
         DevBuf<float> d_alpha_tmp;
         const bool per_voxel_alpha = l1 && sp.alpha_mode != SHOREFIT_ALPHA_FIXED;
         if (per_voxel_alpha && !amap) {
@@ -612,6 +625,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             if (bic) SF_HIP(hipMemsetAsync(bic, 0, sizeof(float) * nvox_total, s));
         }
         if (out->rss_l2 && dev && nvox_total) SF_HIP(hipMemsetAsync(out->rss_l2, 0, sizeof(float) * nvox_total, s));
+        if (rssl2 && nvox_total) SF_HIP(hipMemsetAsync(rssl2, 0, sizeof(float) * nvox_total, s));
 
         // --- design + solver set-up (src/fit.rs:718-759) -------------------
         const int precision = (l1 && sp.max_iter > 0) ? sp.precision : SHOREFIT_PREC_FP32;
@@ -653,12 +667,58 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             L.kind = kind;
             L.alpha_map = amap;
             L.bic = bic;
+            // alpha path: f_i = exp(t_i ln eps), t_i = i / (n - 1), so that
+            // alpha_i = alpha_max f_i (src/solver/alpha.rs:57-69)
+            DevBuf<float> d_pf;
+            DevBuf<double> d_pf64;
+            TikOps tik;
+            if (path) {
+                double eps = sp.path_eps;
+                if (!(eps > 0.0)) eps = sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC ? 1e-3 : 1e-4;
+                const int n = int(sp.path_n_alphas);
+                std::vector<double> f64(n);
+                std::vector<float> f32(n);
+                for (int i = 0; i < n; ++i) {
+                    f64[i] = std::exp(double(i) / double(n - 1) * std::log(eps));
+                    f32[i] = float(f64[i]);
+                }
+                d_pf.alloc(n);
+                d_pf64.alloc(n);
+                SF_HIP(hipMemcpyAsync(d_pf.p, f32.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+                SF_HIP(hipMemcpyAsync(d_pf64.p, f64.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+                SF_HIP(hipStreamSynchronize(s));
+                L.path_n = n;
+                L.path_f = d_pf.p;
+                L.path_f64 = d_pf64.p;
+                L.path_rule = anchored ? 1 : 0;
+                L.slack = sp.slack;
+                L.rss_l2_ref = rssl2;
+                if (anchored && V > 0) {
+                    // the Tikhonov reference fit (src/fit.rs:404-410, 760-761)
+                    std::vector<double> dn, dl;
+                    regularization_diagonals(sp.radial_order, dn, dl);
+                    pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, tik, s);
+                    TikhonovLaunch T{};
+                    T.data = data;
+                    T.vox = d_vox.p;
+                    T.n_vox = V;
+                    T.N = d.N;
+                    T.sig_stride = N;
+                    T.K = d.K;
+                    T.Kp = d.Kp;
+                    T.Np = d.Np;
+                    T.Pfrag = tik.P.p;
+                    T.Afrag = tik.A.p;
+                    T.rss_l2 = rssl2;
+                    launch_tikhonov(T, s);
+                }
+            }
             setup_work(L, work, d, ops, V, s);
             launch_fista_rhs(L, s);
             SF_HIP(hipEventRecord(e0, s));
-            L.event_mid = e1;  // the mixed path records e1 between solve and epilogue
+            L.event_mid = e1;  // recorded right after the solve kernel
             launch_fista(L, s);
-            if (precision == SHOREFIT_PREC_FP32 || V == 0) SF_HIP(hipEventRecord(e1, s));
+            if (V == 0) SF_HIP(hipEventRecord(e1, s));
         } else {
             d.G.alloc(size_t(d.K) * d.K);
             launch_gram(d.A.p, d.N, d.K, d.G.p, s);
@@ -729,6 +789,8 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                 SF_HIP(hipMemcpy(out->regularization_kind, kind, nvox_total, hipMemcpyDeviceToHost));
             if (l1 && out->alpha) SF_HIP(hipMemcpy(out->alpha, amap, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
             if (l1 && out->bic) SF_HIP(hipMemcpy(out->bic, bic, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+            if (anchored && out->rss_l2)
+                SF_HIP(hipMemcpy(out->rss_l2, rssl2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
         }
         warn.to(out);
     });

@@ -125,7 +125,19 @@ struct FistaLaunch {
     uint32_t* itsbuf;       // [n_vox]
     uint8_t* statusbuf;     // [n_vox]
     float* alphabuf;        // [n_vox]
-    void* event_mid;        // optional hipEvent_t recorded between solve and epilogue
+    void* event_mid;        // optional hipEvent_t recorded right after the solve kernel
+    // alpha path strategies (path_n > 0): point i of voxel v is the FISTA
+    // solve at alpha_i = alpha_max(v) path_f[i], warm-started from point i-1;
+    // a selection pass then writes the chosen point's outputs.
+    int path_n;
+    const float* path_f;    // [path_n] eps^(i/(path_n-1)), device
+    const double* path_f64; // [path_n] same in FP64, device
+    int path_rule;          // 0: BIC argmin, 1: L2-anchored
+    double slack;
+    const float* rss_l2_ref;  // L2-anchored: Tikhonov RSS, indexed like the outputs
+    float* Xpath;           // [n_vox][path_n][Kp]
+    uint32_t* itspath;      // [n_vox][path_n]
+    uint8_t* convpath;      // [n_vox][path_n]
 };
 // b = c A^T s and signal means into L.Bbuf / L.meanbuf (FP64 accumulation).
 void launch_fista_rhs(const FistaLaunch& L, hipStream_t s);

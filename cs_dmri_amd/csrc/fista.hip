@@ -75,6 +75,20 @@ struct FistaArgs {
     uint8_t* kind;
     float* alpha_map;
     float* bic;
+    // alpha path (path_n > 0): voxel v solves at alpha_i = alpha_max f_i,
+    // i = 0 .. path_n-1, each solve warm-started from the previous one
+    // (src/solver/fista.rs:165-180); point i of voxel v goes to
+    // Xpath[(v path_n + i) Kp ..], itspath / convpath[v path_n + i].
+    int path_n;
+    const float* path_f;       // [path_n] f_i = eps^(i/(path_n-1))
+    const double* path_f64;    // same in FP64 (reported alphas)
+    const float* mom;          // momentum table (t_k - 1)/t_{k+1}, k < max_iter
+    float* Xpath;
+    uint32_t* itspath;
+    uint8_t* convpath;         // bit0 converged, bit1 all-zero voxel (alpha_max = 0)
+    int path_rule;             // 0: BIC argmin, 1: L2-anchored
+    double slack;
+    const float* rss_l2_ref;   // L2-anchored: Tikhonov RSS, indexed like the outputs
 };
 
 // Prologue: b = c A^T s for the wave's 16 voxels (c = 1/(N L) folded into
@@ -222,7 +236,7 @@ __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data
 
 // Plain FP32 form: V = M Y + B recomputed exactly on FP32 MFMA every
 // iteration. WAVES waves per workgroup, 16 voxels per wave.
-template <int MB, int WAVES, bool M_IN_LDS>
+template <int MB, int WAVES, bool M_IN_LDS, bool PATH>
 __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a) {
     extern __shared__ floatx4 Ms[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -250,20 +264,29 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
 
     // alpha-ratio strategy: alpha_v = ratio * |A^T s|_inf / N, i.e.
     // thr_v = ratio * max|b| (src/solver/alpha.rs:229-235).
-    float thr = a.thr;
+    // Path strategies: thr_i = step alpha_max f_i = max|b| f_i.
+    float thr = a.thr, bmax = 0.f;
     double alpha_v = a.alpha;
     bool zero_voxel = false;
-    if (a.ratio > 0.f) {
-        float m = 0.f;
+    if (PATH || a.ratio > 0.f) {
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) m = fmaxf(m, fabsf(b[mb][r]));
-        m = lane_max4(m);
-        thr = a.ratio * m;
-        alpha_v = double(m) / double(a.step) * double(a.ratio);
-        zero_voxel = (m == 0.f);
+            for (int r = 0; r < 4; ++r) bmax = fmaxf(bmax, fabsf(b[mb][r]));
+        bmax = lane_max4(bmax);
+        zero_voxel = (bmax == 0.f);
+        if constexpr (PATH) {
+            thr = bmax * a.path_f[0];
+        } else {
+            thr = a.ratio * bmax;
+            alpha_v = double(bmax) / double(a.step) * double(a.ratio);
+        }
     }
+    if (PATH && valid && zero_voxel && h == 0)
+        for (int i = 0; i < a.path_n; ++i) {
+            a.itspath[v * a.path_n + i] = 0;
+            a.convpath[v * a.path_n + i] = 3;
+        }
 
     floatx4 x[MB], y[MB];
 #pragma unroll
@@ -273,8 +296,9 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
     uint32_t its = 0;
     double t = 1.0;
     const float tol = a.tol;
+    int pi = 0;  // path point of this slot
 
-    for (uint32_t it = 0; it < a.max_iter; ++it) {
+    for (uint32_t it = 0; PATH || it < a.max_iter; ++it) {
         if (__ballot(active) == 0ull) break;
         // An opaque zero offset keeps the (loop-invariant) M fragment loads
         // inside the loop instead of hoisted into registers for all blocks.
@@ -296,9 +320,16 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
             }
             acc[mb] = c;
         }
-        const double tn = 0.5 * (1.0 + sqrt(1.0 + 4.0 * t * t));
-        const float mom = float((t - 1.0) / tn);
-        t = tn;
+        float mom;
+        if constexpr (PATH) {
+            // per-slot iteration counts differ along the path: the momentum
+            // comes from the table (the same FP64 recurrence, rounded)
+            mom = a.mom[active ? its : 0];
+        } else {
+            const double tn = 0.5 * (1.0 + sqrt(1.0 + 4.0 * t * t));
+            mom = float((t - 1.0) / tn);
+            t = tn;
+        }
         float dd = 0.f, xx = 0.f;
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
@@ -317,7 +348,38 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
             }
         dd = lane_sum4(dd);
         xx = lane_sum4(xx);
-        if (active) {
+        if constexpr (PATH) {
+            bool next = false;
+            if (active) {
+                ++its;
+                const float rel = sqrtf(dd) / fmaxf(sqrtf(xx), 1e-12f);
+                const bool c = rel < tol;
+                if (c || its >= a.max_iter) {
+                    const int64_t pt = v * a.path_n + pi;
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb)
+                        *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = x[mb];
+                    if (h == 0) {
+                        a.itspath[pt] = its;
+                        a.convpath[pt] = c ? 1 : 0;
+                    }
+                    if (pi + 1 < a.path_n) {
+                        // warm start of the next point: x0 = y0 = x, t = 1
+                        ++pi;
+                        thr = bmax * a.path_f[pi];
+                        its = 0;
+                        next = true;
+                    } else {
+                        active = false;
+                    }
+                }
+            }
+            if (__ballot(next) != 0ull) {
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+                    if (next) y[mb] = x[mb];
+            }
+        } else if (active) {
             its = it + 1;
             const float rel = sqrtf(dd) / fmaxf(sqrtf(xx), 1e-12f);
             if (rel < tol) {
@@ -326,6 +388,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
             }
         }
     }
+    if constexpr (PATH) return;  // the selection pass writes the outputs
 
     float rss, sst;
     residual<MB>(a, sig, valid, h, lane, x, mean, rss, sst);
@@ -351,13 +414,14 @@ __global__ void __launch_bounds__(256) tikhonov_kernel(const FistaArgs a, float*
     float rss, sst;
     residual<MB>(a, sig, valid, h, lane, c, mean, rss, sst);
     if (!valid) return;
+    if (a.coef)
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int k = 16 * mb + 4 * h + r;
-            if (k < a.K) a.coef[out_idx * a.K + k] = c[mb][r];
-        }
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * mb + 4 * h + r;
+                if (k < a.K) a.coef[out_idx * a.K + k] = c[mb][r];
+            }
     if (h != 0) return;
     const double ss_res = double(rss);
     if (a.resid) a.resid[out_idx] = float(sqrt(ss_res));
@@ -390,7 +454,7 @@ void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
     const int64_t tiles = (n_vox + 15) / 16;
     const unsigned grid = unsigned((tiles + WAVES - 1) / WAVES);
     const size_t lds = IN_LDS ? size_t(MB) * MB * 64 * 16 : 0;
-    auto kern = fista_f32_kernel<MB, WAVES, IN_LDS>;
+    auto kern = args.path_n > 0 ? fista_f32_kernel<MB, WAVES, IN_LDS, true> : fista_f32_kernel<MB, WAVES, IN_LDS, false>;
     if (lds > 64 * 1024)  // gfx950 has 160 KB of LDS per CU; a refusal surfaces at launch
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int(lds));
@@ -450,6 +514,12 @@ struct MixedArgs {
     uint32_t* its;          // [n_vox]
     uint8_t* status;        // [n_vox] bit0 converged, bit1 all-zero voxel (alpha-ratio)
     float* alpha_v;         // [n_vox] alpha used
+    // alpha path (PATH kernels): as in FistaArgs
+    int path_n;
+    const float* path_f;
+    float* Xpath;
+    uint32_t* itspath;
+    uint8_t* convpath;
 };
 
 template <bool BF16>
@@ -460,6 +530,14 @@ __device__ __forceinline__ floatx4 mfma16(uint4 a, uint4 b, floatx4 c) {
     else
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a), __builtin_bit_cast(half8, b), c,
                                                       0, 0, 0);
+}
+
+template <bool BF16>
+__device__ __forceinline__ float2v unpack2(uint32_t u) {
+    if constexpr (BF16)
+        return __builtin_convertvector(__builtin_bit_cast(bf16x2v, u), float2v);
+    else
+        return __builtin_convertvector(__builtin_bit_cast(half2v, u), float2v);
 }
 
 template <bool BF16>
@@ -482,7 +560,7 @@ __device__ __forceinline__ float pow2_to(float bound, int target) {
 // registers (order 10, where the FP16 matrix alone takes 128 KB of LDS).
 // SPLIT: refresh with FP16 hi/lo splits (three FP16 MFMA passes) instead of
 // the FP32 MFMA (FP16 increments only).
-template <int MB, int WAVES, bool BF16, bool B_LDS, bool NONNEG, bool SPLIT>
+template <int MB, int WAVES, bool BF16, bool B_LDS, bool NONNEG, bool SPLIT, bool PATH>
 __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs a) {
     constexpr int KC = MB / 2;
     constexpr int BSTRIDE = 16 * MB + 4;  // padded slot stride (floats) against LDS bank conflicts
@@ -500,6 +578,8 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     float thr_u = a.thr, thr = a.thr, sc = 1.f, mprev = 0.f, dprev = 0.f, mom = 0.f;
     float xref = 0.f, drift = 0.f;  // |x| at its last evaluation, sum of |dx| since
     bool alive = false, zero = false;
+    int pi = 0;        // path point of this slot (PATH)
+    float bmx = 0.f;   // max|b| of this slot's voxel (PATH)
 
     // Claims new voxels for the slots flagged in `need` (one atomic per
     // wave) and loads them: x = y = 0, v = b 2^e. An all-zero voxel of the
@@ -542,8 +622,16 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
         m = lane_max4(m);  // alpha-ratio: thr_v = ratio * max|b| (src/solver/alpha.rs:229-235)
         n2 = lane_sum4(n2);
         if (need) {
-            thr_u = a.ratio > 0.f ? a.ratio * m : a.thr;
-            zero = alive && a.ratio > 0.f && m == 0.f;
+            if constexpr (PATH) {
+                // path point i: thr = step alpha_max f_i = max|b| f_i
+                pi = 0;
+                bmx = m;
+                thr_u = m * a.path_f[0];
+                zero = alive && m == 0.f;
+            } else {
+                thr_u = a.ratio > 0.f ? a.ratio * m : a.thr;
+                zero = alive && a.ratio > 0.f && m == 0.f;
+            }
             // the first steps move x by about |b|
             sc = pow2_to(sqrtf(n2), 10);
             thr = thr_u * sc;
@@ -636,8 +724,46 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                 conv = cv;
                 done = conv || it >= a.max_iter;
             }
+            bool next = false;  // PATH: this slot moves on to its next path point
+            if constexpr (PATH) {
+                next = done && pi + 1 < a.path_n;
+                if (next) done = false;
+                if (__ballot(next) != 0ull) {
+                    if (next) {
+                        const float isc = 1.f / sc;
+                        const int64_t pt = vid * a.path_n + pi;
+#pragma unroll
+                        for (int mb = 0; mb < MB; ++mb)
+                            *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
+                        if (h == 0) {
+                            a.itspath[pt] = zero ? 0u : it;
+                            a.convpath[pt] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+                        }
+                        ++pi;
+                        thr_u = bmx * a.path_f[pi];
+                        thr = thr_u * sc;
+                        it = 0;
+                    }
+                    // Warm start x0 = y0 = x, t = 1 (src/solver/fista.rs:76-80,
+                    // 165-180): the increment takes y from y_k to x instead of
+                    // to the extrapolated point, so dy loses y - x.
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int mb = 2 * kc + (q >> 1), r0 = 2 * (q & 1);
+                            float2v t2 = unpack2<BF16>(dyh[kc][q]);
+                            t2[0] -= y[mb][r0] - x[mb][r0];
+                            t2[1] -= y[mb][r0 + 1] - x[mb][r0 + 1];
+                            if (next) dyh[kc][q] = pack2<BF16>(t2[0], t2[1]);
+                        }
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb)
+                        if (next) y[mb] = x[mb];
+                }
+            }
             const float bound = (1.f + mom) * nd + mprev * dprev;
-            mprev = mom;
+            mprev = next ? 0.f : mom;
             dprev = nd;
             // next iteration's momentum, fetched while the MFMA phase runs
             // (it < max_iter for every slot that is still running)
@@ -776,13 +902,24 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
         }
         if (done) {
             const float isc = 1.f / sc;  // exact: sc is a power of two
+            if constexpr (PATH) {
+                const int64_t pt = vid * a.path_n + pi;
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb)
-                *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
-            if (h == 0) {
-                a.its[vid] = zero ? 0u : it;
-                a.status[vid] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
-                a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
+                for (int mb = 0; mb < MB; ++mb)
+                    *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
+                if (h == 0) {
+                    a.itspath[pt] = zero ? 0u : it;
+                    a.convpath[pt] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+                }
+            } else {
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+                    *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
+                if (h == 0) {
+                    a.its[vid] = zero ? 0u : it;
+                    a.status[vid] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+                    a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
+                }
             }
         }
         refill(done);
@@ -812,6 +949,98 @@ __global__ void __launch_bounds__(256) fista_epilogue_kernel(const FistaArgs a, 
     write_voxel<MB>(a, out_idx, h, x, rss, sst, its[v], st & 1, double(alpha_v[v]), (st & 2) != 0);
 }
 
+// Selection pass of the alpha-path strategies (src/solver/alpha.rs:107-165,
+// 244-263, 285-316): per voxel, the residual sum of squares and non-zero count
+// of every path point, then the BIC argmin (ties to the larger alpha) or the
+// L2-anchored rule (largest alpha with RSS <= (1 + slack) RSS_L2, else the
+// RSS argmin), and the chosen point's outputs.
+template <int MB>
+__global__ void __launch_bounds__(256) path_select_kernel(const FistaArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    const int64_t v = (int64_t(blockIdx.x) * 4 + wave) * 16 + j;
+    const bool valid = v < a.n_vox;
+    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
+    const float* sig = a.data + out_idx * a.sig_stride;
+    const int P = a.path_n;
+    const float mean = valid ? a.mean[v] : 0.f;
+    const bool zero = valid && (a.convpath[v * P] & 2) != 0;
+    const bool live = valid && !zero;
+    const double n = double(a.N), ln_n = log(n);
+    const double thr_l2 = (a.path_rule == 1 && valid) ? (1.0 + a.slack) * double(a.rss_l2_ref[out_idx]) : 0.0;
+    double best = INFINITY, best_rss = INFINITY;
+    int idx = 0, idx_min = 0;
+    float rss_idx = 0.f, rss_min = 0.f, sst = 0.f;
+    bool found = false;
+    for (int i = 0; i < P; ++i) {
+        floatx4 x[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+            x[mb] = live ? *reinterpret_cast<const floatx4*>(a.Xpath + (v * P + i) * a.Kp + 16 * mb + 4 * h)
+                         : floatx4{0.f, 0.f, 0.f, 0.f};
+        float rss;
+        residual<MB>(a, sig, valid, h, lane, x, mean, rss, sst);
+        int nnz = 0;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) nnz += (x[mb][r] != 0.f);
+        nnz += __shfl_xor(nnz, 16);
+        nnz += __shfl_xor(nnz, 32);
+        const double rs = double(rss);
+        if (a.path_rule == 0) {
+            const double b = n * log(fmax(rs, 2.2250738585072014e-308) / n) + ln_n * double(nnz);
+            if (b < best) {
+                best = b;
+                idx = i;
+                rss_idx = rss;
+            }
+        } else {
+            if (!found && rs <= thr_l2) {
+                found = true;
+                idx = i;
+                rss_idx = rss;
+            }
+            if (rs < best_rss) {
+                best_rss = rs;
+                idx_min = i;
+                rss_min = rss;
+            }
+        }
+    }
+    if (a.path_rule == 1 && !found) {
+        idx = idx_min;
+        rss_idx = rss_min;
+    }
+    if (!valid) return;
+    floatx4 x[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+        x[mb] = live ? *reinterpret_cast<const floatx4*>(a.Xpath + (v * P + idx) * a.Kp + 16 * mb + 4 * h)
+                     : floatx4{0.f, 0.f, 0.f, 0.f};
+    // alpha_i = alpha_max f_i with alpha_max = max|b| / step
+    float bm = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const floatx4 bq = *reinterpret_cast<const floatx4*>(a.B + v * a.Kp + 16 * mb + 4 * h);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bm = fmaxf(bm, fabsf(bq[r]));
+    }
+    bm = fmaxf(bm, __shfl_xor(bm, 16));
+    bm = fmaxf(bm, __shfl_xor(bm, 32));
+    const double alpha_i = double(bm) / double(a.step) * a.path_f64[idx];
+    const int64_t pt = v * P + idx;
+    write_voxel<MB>(a, out_idx, h, x, rss_idx, sst, zero ? 0u : a.itspath[pt], (a.convpath[pt] & 1) != 0,
+                    zero ? 0.0 : alpha_i, zero);
+}
+
+template <int MB>
+void launch_path_select(const FistaArgs& a, hipStream_t s) {
+    const int64_t tiles = (a.n_vox + 15) / 16;
+    hipLaunchKernelGGL(path_select_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, a);
+    SF_HIP(hipGetLastError());
+}
+
 template <int MB, bool BF16>
 void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     constexpr bool B_LDS = MB <= 12;
@@ -839,10 +1068,18 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.its = L.itsbuf;
     m.status = L.statusbuf;
     m.alpha_v = L.alphabuf;
+    m.path_n = L.path_n;
+    m.path_f = L.path_f;
+    m.Xpath = L.Xpath;
+    m.itspath = L.itspath;
+    m.convpath = L.convpath;
+    const bool path = L.path_n > 0;
     const size_t lds = size_t(MB) * KC * 64 * 16 + (B_LDS ? size_t(WAVES) * 16 * (16 * MB + 4) * 4 : 0);
     constexpr bool SPLIT = !BF16;
-    auto kern = L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true, SPLIT>
-                         : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false, SPLIT>;
+    auto kern = path ? (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true, SPLIT, true>
+                                 : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false, SPLIT, true>)
+                     : (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true, SPLIT, false>
+                                 : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false, SPLIT, false>);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(lds));
     int dev = 0, cus = 0, per_cu = 0;
@@ -857,6 +1094,10 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(64 * WAVES), lds, s, m);
     SF_HIP(hipGetLastError());
     if (L.event_mid) SF_HIP(hipEventRecord(static_cast<hipEvent_t>(L.event_mid), s));
+    if (path) {
+        launch_path_select<MB>(ea, s);
+        return;
+    }
     const int64_t tiles = (L.n_vox + 15) / 16;
     hipLaunchKernelGGL(fista_epilogue_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, ea, L.Xbuf,
                        L.itsbuf, L.statusbuf, L.alphabuf);
@@ -909,6 +1150,16 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
     a.B = L.Bbuf;
     a.mean = L.meanbuf;
     a.Kp = L.Kp;
+    a.path_n = L.path_n;
+    a.path_f = L.path_f;
+    a.path_f64 = L.path_f64;
+    a.mom = L.mom;
+    a.Xpath = L.Xpath;
+    a.itspath = L.itspath;
+    a.convpath = L.convpath;
+    a.path_rule = L.path_rule;
+    a.slack = L.slack;
+    a.rss_l2_ref = L.rss_l2_ref;
     if (L.precision != SHOREFIT_PREC_FP32) {
         const bool bf = L.precision == SHOREFIT_PREC_BF16;
         switch (L.Kp / 16) {
@@ -928,16 +1179,14 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
         return;
     }
     switch (L.Kp / 16) {
-        case 1: launch_f32<1>(a, L.n_vox, s); break;
-        case 2: launch_f32<2>(a, L.n_vox, s); break;
-        case 3: launch_f32<3>(a, L.n_vox, s); break;
-        case 4: launch_f32<4>(a, L.n_vox, s); break;
-        case 5: launch_f32<5>(a, L.n_vox, s); break;
-        case 6: launch_f32<6>(a, L.n_vox, s); break;
-        case 8: launch_f32<8>(a, L.n_vox, s); break;
-        case 10: launch_f32<10>(a, L.n_vox, s); break;
-        case 12: launch_f32<12>(a, L.n_vox, s); break;
-        case 16: launch_f32<16>(a, L.n_vox, s); break;
+#define SF_F32(MB)                                                  \
+    case MB:                                                        \
+        launch_f32<MB>(a, L.n_vox, s);                              \
+        if (L.event_mid) SF_HIP(hipEventRecord(static_cast<hipEvent_t>(L.event_mid), s)); \
+        if (L.path_n > 0) launch_path_select<MB>(a, s);             \
+        break;
+        SF_F32(1) SF_F32(2) SF_F32(3) SF_F32(4) SF_F32(5) SF_F32(6) SF_F32(8) SF_F32(10) SF_F32(12) SF_F32(16)
+#undef SF_F32
         default:
             throw Error{SHOREFIT_E_UNSUPPORTED,
                         "FISTA kernel: padded coefficient count " + std::to_string(L.Kp) + " is not instantiated"};

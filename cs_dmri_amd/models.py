@@ -183,10 +183,12 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
     kind = np.zeros(shp, np.uint8) if diag else None
     amap = np.zeros(shp, np.float32) if (diag and l1) else None
     bic = np.zeros(shp, np.float32) if (diag and l1) else None
+    # the Tikhonov reference RSS map of the L2-anchored strategy (src/fit.rs:384-386)
+    rss_l2 = np.zeros(shp, np.float32) if (diag and l1 and alpha_mode == "l2-anchored") else None
     out = L.ShorefitOutput()
     out.coefficients = L.ptr(coefs).value
     for name, arr in (("r2", r2), ("residual_l2", res), ("iterations", its), ("regularization_kind", kind),
-                      ("alpha", amap), ("bic", bic)):
+                      ("alpha", amap), ("bic", bic), ("rss_l2", rss_l2)):
         setattr(out, name, None if arr is None else arr.ctypes.data)
     v = L.ShorefitVolume(vol.ctypes.data, m.ctypes.data, nx, ny, nz, 0, 0)
     L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), None))
@@ -209,4 +211,6 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
         if l1:
             result["alpha"] = amap
             result["bic"] = bic
+        if rss_l2 is not None:
+            result["rss_l2"] = rss_l2
     return result
