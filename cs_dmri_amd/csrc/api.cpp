@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -24,6 +25,94 @@ constexpr size_t kMinShoreShells = 2;        // src/fit.rs:614
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+bool& stream_scope_active() {
+    thread_local bool a = false;
+    return a;
+}
+
+namespace {
+struct BlockCache {
+    std::map<int, std::multimap<size_t, void*>> free;  // device -> size -> block
+    std::map<void*, std::pair<int, size_t>> size_of;   // block -> (device, bytes)
+    std::vector<void*> parked;                         // released during the current call
+    size_t cached_bytes = 0;
+    ~BlockCache() {
+        for (auto& kv : size_of) (void)hipFree(kv.first);
+    }
+};
+BlockCache& block_cache() {
+    thread_local BlockCache c;
+    return c;
+}
+// Blocks are rounded up to 2 MiB (small ones to 4 KiB) so that sizes recur.
+size_t round_block(size_t bytes) {
+    const size_t g = bytes >= (size_t(1) << 20) ? (size_t(2) << 20) : 4096;
+    return (bytes + g - 1) / g * g;
+}
+// Cached (unused) bytes kept per thread beyond which free blocks go back to
+// the driver.
+constexpr size_t kCacheKeep = size_t(64) << 30;
+}  // namespace
+
+void* cache_alloc(size_t bytes) {
+    BlockCache& c = block_cache();
+    int dev = 0;
+    SF_HIP(hipGetDevice(&dev));
+    const size_t need = round_block(bytes);
+    auto& fl = c.free[dev];
+    auto it = fl.lower_bound(need);
+    if (it != fl.end() && it->first <= 2 * need) {
+        void* p = it->second;
+        c.cached_bytes -= it->first;
+        fl.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) {
+        // give the cached blocks of this device back and retry once
+        (void)hipGetLastError();
+        SF_HIP(hipDeviceSynchronize());
+        for (auto& kv : fl) {
+            c.cached_bytes -= kv.first;
+            c.size_of.erase(kv.second);
+            (void)hipFree(kv.second);
+        }
+        fl.clear();
+        SF_HIP(hipMalloc(&p, need));
+    }
+    c.size_of[p] = {dev, need};
+    return p;
+}
+
+void cache_release(void* p) { block_cache().parked.push_back(p); }
+
+StreamScope::StreamScope(hipStream_t stream) : s(stream), prev_active(stream_scope_active()) {
+    stream_scope_active() = true;
+}
+
+StreamScope::~StreamScope() {
+    stream_scope_active() = prev_active;
+    if (prev_active) return;  // an enclosing scope parks and recycles
+    BlockCache& c = block_cache();
+    if (c.parked.empty()) return;
+    // The call's queued work may still read parked blocks (also on an error
+    // path): wait for it before they become reusable.
+    (void)hipStreamSynchronize(s);
+    for (void* p : c.parked) {
+        auto it = c.size_of.find(p);
+        if (it == c.size_of.end()) continue;
+        if (c.cached_bytes + it->second.second > kCacheKeep) {
+            (void)hipFree(p);
+            c.size_of.erase(it);
+            continue;
+        }
+        c.free[it->second.first].emplace(it->second.second, p);
+        c.cached_bytes += it->second.second;
+    }
+    c.parked.clear();
+}
 
 namespace {
 
@@ -357,6 +446,7 @@ int shorefit_design_matrix(const ShorefitGradients* g, uint32_t radial_order, do
         if (!g || !A_out) throw Error{SHOREFIT_E_INVALID, "null argument"};
         double big = g->big_delta, small = g->small_delta;
         if (std::isnan(big) || std::isnan(small)) tortoise_deltas(g->bvals, g->n_grads, &big, &small);
+        StreamScope scope(nullptr);
         Design d;
         build_design(d, *g, big, small, radial_order, zeta, nullptr);
         SF_HIP(hipMemcpy(A_out, d.A.p, sizeof(double) * d.N * d.K, hipMemcpyDeviceToHost));
@@ -371,6 +461,7 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         if (!A || !signals || !coef || n_rows <= 0 || n_cols <= 0 || n_signals < 0)
             throw Error{SHOREFIT_E_INVALID, "invalid fista_batch arguments"};
         hipStream_t s = nullptr;
+        StreamScope scope(s);
         if (precision != SHOREFIT_PREC_FP32 && precision != SHOREFIT_PREC_FP16 && precision != SHOREFIT_PREC_BF16)
             throw Error{SHOREFIT_E_INVALID, "precision must be fp32, fp16 or bf16"};
         if (max_iter == 0) precision = SHOREFIT_PREC_FP32;
@@ -431,6 +522,7 @@ int shorefit_tikhonov_batch(const double* A, int32_t n_rows, int32_t n_cols, con
         if (!A || !signals || !coef || n_rows <= 0 || n_cols <= 0 || n_signals < 0)
             throw Error{SHOREFIT_E_INVALID, "invalid tikhonov_batch arguments"};
         hipStream_t s = nullptr;
+        StreamScope scope(s);
         Design d;
         upload_design(d, A, n_rows, n_cols, s);
         d.G.alloc(size_t(d.K) * d.K);
@@ -467,6 +559,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         if (!grads || !vol || !spec || !out) throw Error{SHOREFIT_E_INVALID, "null argument"};
         if (!out->coefficients) throw Error{SHOREFIT_E_INVALID, "coefficients output is required"};
         hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
         const ShorefitSpec& sp = *spec;
         Warnings warn;
 
@@ -555,19 +648,26 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                                 "are not meaningful; use a lower radial order or L1");
 
         // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
+        // A device mask is compacted on the device; a host mask on the host;
+        // without a mask every voxel is fitted in place (no index list).
         const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
-        std::vector<uint8_t> mask_h;
-        const uint8_t* mask = vol->mask;
-        if (mask && vol->on_device) {
-            mask_h.resize(nvox_total);
-            if (nvox_total) SF_HIP(hipMemcpy(mask_h.data(), mask, nvox_total, hipMemcpyDeviceToHost));
-            mask = mask_h.data();
+        DevBuf<int64_t> d_vox;
+        int64_t V = 0;
+        if (!vol->mask) {
+            V = nvox_total;
+        } else if (vol->on_device) {
+            V = compact_mask(vol->mask, nvox_total, d_vox, s);
+        } else {
+            std::vector<int64_t> vox;
+            vox.reserve(nvox_total);
+            for (int64_t i = 0; i < nvox_total; ++i)
+                if (vol->mask[i]) vox.push_back(i);
+            V = int64_t(vox.size());
+            d_vox.alloc(V);
+            if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
+            SF_HIP(hipStreamSynchronize(s));  // vox is released at scope exit
         }
-        std::vector<int64_t> vox;
-        vox.reserve(nvox_total);
-        for (int64_t i = 0; i < nvox_total; ++i)
-            if (!mask || mask[i]) vox.push_back(i);
-        const int64_t V = int64_t(vox.size());
+        const int64_t* vox_dev = d_vox.p;  // null: voxel v is row v
         out->n_voxels = V;
 
         // --- device buffers -------------------------------------------------
@@ -575,7 +675,6 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         DevBuf<float> d_data, d_coef, d_r2, d_res, d_alpha, d_bic, d_rssl2;
         DevBuf<uint32_t> d_it;
         DevBuf<uint8_t> d_kind;
-        DevBuf<int64_t> d_vox(V);
         const bool dev = vol->on_device != 0;
         const float* data = vol->data;
         if (!dev) {
@@ -584,7 +683,6 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                 SF_HIP(hipMemcpyAsync(d_data.p, vol->data, sizeof(float) * nvox_total * N, hipMemcpyHostToDevice, s));
             data = d_data.p;
         }
-        if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
         auto out_buf = [&](auto* user, auto& buf, size_t count) {
             using T = std::remove_pointer_t<decltype(user)>;
             if (!user) return static_cast<T*>(nullptr);
@@ -642,7 +740,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             FistaLaunch L{};
             FistaWork work;
             L.data = data;
-            L.vox = d_vox.p;
+            L.vox = vox_dev;
             L.n_vox = V;
             L.N = d.N;
             L.sig_stride = N;
@@ -700,7 +798,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                     pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, tik, s);
                     TikhonovLaunch T{};
                     T.data = data;
-                    T.vox = d_vox.p;
+                    T.vox = vox_dev;
                     T.n_vox = V;
                     T.N = d.N;
                     T.sig_stride = N;
@@ -728,7 +826,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, ops, s);
             TikhonovLaunch L{};
             L.data = data;
-            L.vox = d_vox.p;
+            L.vox = vox_dev;
             L.n_vox = V;
             L.N = d.N;
             L.sig_stride = N;
@@ -756,20 +854,24 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         // --- alpha distribution (src/fit.rs:334-352) ------------------------
         out->has_alpha_distribution = 0;
         if (per_voxel_alpha) {
-            std::vector<float> a_all(nvox_total);
-            if (nvox_total)
-                SF_HIP(hipMemcpy(a_all.data(), amap, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
-            std::vector<double> seen;
-            seen.reserve(V);
-            for (int64_t i : vox) seen.push_back(double(a_all[i]));
-            std::sort(seen.begin(), seen.end());
+            // the masked voxels' alphas in work order, then three order
+            // statistics (the values a full sort would put at those ranks)
+            std::vector<float> a_v(V);
+            if (V) {
+                DevBuf<float> d_av(V);
+                launch_gather_f32(amap, vox_dev, V, d_av.p, s);
+                SF_HIP(hipMemcpyAsync(a_v.data(), d_av.p, sizeof(float) * V, hipMemcpyDeviceToHost, s));
+                SF_HIP(hipStreamSynchronize(s));
+            }
+            std::vector<double> seen(a_v.begin(), a_v.end());
             out->has_alpha_distribution = 1;
             if (seen.empty()) {
                 out->alpha_distribution[0] = out->alpha_distribution[1] = out->alpha_distribution[2] = 0.0;
             } else {
                 auto pct = [&](double q) {
-                    size_t i = size_t(std::llround(double(seen.size() - 1) * q));
-                    return seen[std::min(i, seen.size() - 1)];
+                    size_t i = std::min(size_t(std::llround(double(seen.size() - 1) * q)), seen.size() - 1);
+                    std::nth_element(seen.begin(), seen.begin() + i, seen.end());
+                    return seen[i];
                 };
                 out->alpha_distribution[0] = pct(0.5);
                 out->alpha_distribution[1] = pct(0.10);

@@ -26,22 +26,54 @@ struct Error {
             throw ::shorefit::Error{SHOREFIT_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
     } while (0)
 
+// Device memory of the entry points. While a StreamScope is active (one per
+// entry-point call, on the calling thread), DevBuf blocks come from a
+// per-thread, per-device cache of earlier calls' blocks; a block released
+// during the call is only parked, and becomes reusable when the scope closes,
+// after the call's stream has been synchronized, so no block is handed out
+// while queued work may still use it. Repeated fits therefore allocate no
+// device memory after the first call. Outside a StreamScope, DevBuf uses
+// hipMalloc / hipFree directly.
+void* cache_alloc(size_t bytes);
+void cache_release(void* p);
+bool& stream_scope_active();
+
+struct StreamScope {
+    hipStream_t s;
+    bool prev_active;
+    explicit StreamScope(hipStream_t stream);
+    ~StreamScope();
+};
+
 // Owning device buffer (freed on scope exit; errors on free are ignored).
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool cached = false;
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) SF_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        if (!count) return;
+        if (stream_scope_active()) {
+            p = static_cast<T*>(cache_alloc(count * sizeof(T)));
+            cached = true;
+        } else {
+            SF_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        }
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (cached)
+                cache_release(p);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        cached = false;
     }
     ~DevBuf() { release(); }
     DevBuf(const DevBuf&) = delete;
@@ -54,6 +86,12 @@ int n_coeffs(uint32_t radial_order);
 // Device FP64 design matrix (N x K row-major) from q-magnitudes and bvecs.
 void launch_design_matrix(const double* d_q, const double* d_bvecs, int N, uint32_t radial_order, double zeta,
                           double* d_A, hipStream_t s);
+
+// Masked voxels (mask != 0, linear order) of a device mask of n voxels into
+// vox (allocated here, n entries); returns their count.
+int64_t compact_mask(const uint8_t* d_mask, int64_t n, DevBuf<int64_t>& vox, hipStream_t s);
+// dst[i] = src[idx[i]] (src[i] when idx is null), i < n.
+void launch_gather_f32(const float* d_src, const int64_t* d_idx, int64_t n, float* d_dst, hipStream_t s);
 
 // ------------------------------------------------------------ FP64 setup ----
 // G = A^T A (K x K), A row-major N x K.

@@ -360,7 +360,107 @@ __global__ void pad_cols_kernel(const double* __restrict__ src, int rows, int co
 
 inline unsigned blocks_for(int64_t n, int b) { return unsigned((n + b - 1) / b); }
 
+// ------------------------------------------------------ mask compaction ----
+// Masked voxels in linear (x -> y -> z) order (src/voxel_loop.rs:28-42),
+// on the device: per-tile counts, one scan, then an ordered write per tile.
+constexpr int kMaskTile = 4096;  // voxels per workgroup: 256 threads x 16 rounds
+
+__global__ void __launch_bounds__(256) mask_count_kernel(const uint8_t* __restrict__ mask, int64_t n,
+                                                         uint32_t* __restrict__ counts) {
+    const int64_t base = int64_t(blockIdx.x) * kMaskTile;
+    int c = 0;
+    for (int i = 0; i < kMaskTile / 256; ++i) {
+        const int64_t v = base + i * 256 + threadIdx.x;
+        c += (v < n && mask[v] != 0) ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    __shared__ int ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = uint32_t(ws[0] + ws[1] + ws[2] + ws[3]);
+}
+
+// Exclusive scan of the tile counts in one workgroup; the total goes to *total.
+__global__ void __launch_bounds__(1024) mask_scan_kernel(const uint32_t* __restrict__ counts, int64_t nb,
+                                                         int64_t* __restrict__ offsets, int64_t* __restrict__ total) {
+    __shared__ int64_t part[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t b0 = int64_t(threadIdx.x) * per, b1 = b0 + per < nb ? b0 + per : nb;
+    int64_t sum = 0;
+    for (int64_t b = b0; b < b1; ++b) sum += counts[b];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < 1024; ++t) {
+            const int64_t x = part[t];
+            part[t] = run;
+            run += x;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t b = b0; b < b1; ++b) {
+        offsets[b] = run;
+        run += counts[b];
+    }
+}
+
+__global__ void __launch_bounds__(256) mask_write_kernel(const uint8_t* __restrict__ mask, int64_t n,
+                                                         const int64_t* __restrict__ offsets,
+                                                         int64_t* __restrict__ vox) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = int64_t(blockIdx.x) * kMaskTile;
+    int64_t off = offsets[blockIdx.x];
+    for (int i = 0; i < kMaskTile / 256; ++i) {
+        const int64_t v = base + i * 256 + threadIdx.x;
+        const bool f = v < n && mask[v] != 0;
+        const unsigned long long bal = __ballot(f);
+        const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(bal);
+        __syncthreads();
+        int wpre = 0;
+        for (int w = 0; w < wave; ++w) wpre += wsum[w];
+        const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (f) vox[off + wpre + pre] = v;
+        off += tot;
+        __syncthreads();
+    }
+}
+
+__global__ void gather_f32_kernel(const float* __restrict__ src, const int64_t* __restrict__ idx, int64_t n,
+                                  float* __restrict__ dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx ? idx[i] : i];
+}
+
 }  // namespace
+
+int64_t compact_mask(const uint8_t* d_mask, int64_t n, DevBuf<int64_t>& vox, hipStream_t s) {
+    if (n <= 0) return 0;
+    const int64_t nb = (n + kMaskTile - 1) / kMaskTile;
+    DevBuf<uint32_t> counts(nb);
+    DevBuf<int64_t> offsets(nb), total(1);
+    vox.alloc(n);  // upper bound: no round trip before the write pass
+    hipLaunchKernelGGL(mask_count_kernel, dim3(unsigned(nb)), dim3(256), 0, s, d_mask, n, counts.p);
+    SF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mask_scan_kernel, dim3(1), dim3(1024), 0, s, counts.p, nb, offsets.p, total.p);
+    SF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mask_write_kernel, dim3(unsigned(nb)), dim3(256), 0, s, d_mask, n, offsets.p, vox.p);
+    SF_HIP(hipGetLastError());
+    int64_t V = 0;
+    SF_HIP(hipMemcpyAsync(&V, total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    return V;
+}
+
+void launch_gather_f32(const float* d_src, const int64_t* d_idx, int64_t n, float* d_dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gather_f32_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, d_src, d_idx, n, d_dst);
+    SF_HIP(hipGetLastError());
+}
 
 void launch_design_matrix(const double* d_q, const double* d_bvecs, int N, uint32_t R, double zeta, double* d_A,
                           hipStream_t s) {
