@@ -236,18 +236,6 @@ void build_design(Design& d, const ShorefitGradients& g, double big, double smal
     SF_HIP(hipStreamSynchronize(s));  // dq / dv are released at scope exit
 }
 
-void gram_and_lipschitz(Design& d, hipStream_t s) {
-    d.G.alloc(size_t(d.K) * d.K);
-    launch_gram(d.A.p, d.N, d.K, d.G.p, s);
-    DevBuf<double> lam(1);
-    launch_power_iteration(d.G.p, d.K, lam.p, s);
-    double lh = 0.0;
-    SF_HIP(hipMemcpyAsync(&lh, lam.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    SF_HIP(hipStreamSynchronize(s));
-    // src/solver/fista.rs:51,58 — L = sigma_max^2 / n, floored at f64::EPSILON.
-    d.L = std::max(std::max(lh, 0.0) / double(d.N), std::numeric_limits<double>::epsilon());
-}
-
 struct FistaOps {
     DevBuf<float> M, A;
     DevBuf<double> Ap;  // FP64 design zero-padded to N x Kp
@@ -255,56 +243,100 @@ struct FistaOps {
     double step = 0.0;
 };
 
-void pack_fista(const Design& d, FistaOps& ops, int precision, hipStream_t s) {
+// Device buffers of one FISTA solve: right-hand sides, the momentum table,
+// the Lipschitz scalars, and for the mixed-precision persistent kernel its
+// work counter and work-order results (and the alpha-path points).
+struct FistaWork {
+    DevBuf<float> B, mean, mom, X, alpha, Xpath;
+    DevBuf<double> lam, sc;
+    DevBuf<unsigned int> counter;
+    DevBuf<uint32_t> its, itspath;
+    DevBuf<uint8_t> status, convpath;
+};
+
+// A small pinned host buffer per thread for device scalars read mid-call.
+double* pinned_scalars() {
+    struct Pinned {
+        double* p = nullptr;
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    thread_local Pinned h;
+    if (!h.p) SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&h.p), 4 * sizeof(double)));
+    return h.p;
+}
+
+// FISTA set-up (src/solver/fista.rs:38-62) and right-hand sides. The caller
+// fills L's signal fields (data, vox, n_vox, N, sig_stride, K, Kp, Np),
+// precision, max_iter and path_n. In device order: G = A^T A, its largest
+// eigenvalue, L and c = 1 / (L N) on the device, the FP64 right-hand sides
+// b = c A^T s, then the operands that need c on the host. The host waits for
+// L on an event while the right-hand-side kernel runs.
+void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStream_t s) {
+    const int64_t V = L.n_vox;
+    const bool table = L.precision != SHOREFIT_PREC_FP32 || L.path_n > 0;
+    if (table) {
+        // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112), in FP64,
+        // uploaded first, while the stream holds little work
+        std::vector<float> mom(std::max<uint32_t>(L.max_iter, 1));
+        double t = 1.0;
+        for (size_t k = 0; k < mom.size(); ++k) {
+            const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+            mom[k] = float((t - 1.0) / tn);
+            t = tn;
+        }
+        w.mom.alloc(mom.size());
+        SF_HIP(hipMemcpyAsync(w.mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
+        SF_HIP(hipStreamSynchronize(s));  // the host table goes out of scope
+        L.mom = w.mom.p;
+    }
+    d.G.alloc(size_t(d.K) * d.K);
+    launch_gram(d.A.p, d.N, d.K, d.G.p, s);
+    w.lam.alloc(1);
+    w.sc.alloc(2);
+    launch_power_iteration(d.G.p, d.K, w.lam.p, s);
+    launch_fit_scalars(w.lam.p, d.N, w.sc.p, s);
+    double* hs = pinned_scalars();
+    SF_HIP(hipMemcpyAsync(hs, w.sc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    hipEvent_t ev;
+    SF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    struct EventGuard {
+        hipEvent_t e;
+        ~EventGuard() { (void)hipEventDestroy(e); }
+    } guard{ev};
+    SF_HIP(hipEventRecord(ev, s));
+    ops.Ap.alloc(size_t(d.N) * d.Kp);
+    launch_pad_cols(d.A.p, d.N, d.K, d.Kp, ops.Ap.p, s);
+    w.B.alloc(size_t(V) * d.Kp);
+    w.mean.alloc(V);
+    L.A64p = ops.Ap.p;
+    L.c_dev = w.sc.p + 1;
+    L.Bbuf = w.B.p;
+    L.meanbuf = w.mean.p;
+    launch_fista_rhs(L, s);
+    SF_HIP(hipEventSynchronize(ev));
+    d.L = hs[0];
+    // operands that take c on the host
     ops.step = 1.0 / d.L;
     const double c = ops.step / double(d.N);
     ops.M.alloc(size_t(d.Kp) * d.Kp);
     ops.A.alloc(size_t(d.Np) * d.Kp);
     launch_pack_M(d.G.p, d.K, d.Kp, c, ops.M.p, s);
     launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
-    ops.Ap.alloc(size_t(d.N) * d.Kp);
-    launch_pad_cols(d.A.p, d.N, d.K, d.Kp, ops.Ap.p, s);
-    if (precision != SHOREFIT_PREC_FP32) {
+    if (L.precision != SHOREFIT_PREC_FP32) {
         ops.Mlow.alloc(size_t(d.Kp) * d.Kp);
-        launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, precision == SHOREFIT_PREC_BF16, ops.Mlow.p, s);
-        if (precision == SHOREFIT_PREC_FP16) {
+        launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, L.precision == SHOREFIT_PREC_BF16, ops.Mlow.p, s);
+        if (L.precision == SHOREFIT_PREC_FP16) {
             ops.Mlow2.alloc(size_t(d.Kp) * d.Kp);
             launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, false, ops.Mlow2.p, s, 1);
         }
     }
-}
-
-// Device buffers of one FISTA solve: right-hand sides, and for the
-// mixed-precision persistent kernel its momentum table, work counter and
-// work-order results.
-struct FistaWork {
-    DevBuf<float> B, mean, mom, X, alpha, Xpath;
-    DevBuf<unsigned int> counter;
-    DevBuf<uint32_t> its, itspath;
-    DevBuf<uint8_t> status, convpath;
-};
-
-void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& ops, int64_t V, hipStream_t s) {
-    w.B.alloc(size_t(V) * d.Kp);
-    w.mean.alloc(V);
-    L.A64p = ops.Ap.p;
-    L.c = ops.step / double(d.N);
-    L.Bbuf = w.B.p;
-    L.meanbuf = w.mean.p;
+    L.Mfrag = ops.M.p;
+    L.Afrag = ops.A.p;
     L.Mlow = ops.Mlow.p;
     L.Mlow2 = ops.Mlow2.p;
-    if (L.precision == SHOREFIT_PREC_FP32 && L.path_n == 0) return;
-    // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112), in FP64
-    std::vector<float> mom(std::max<uint32_t>(L.max_iter, 1));
-    double t = 1.0;
-    for (size_t k = 0; k < mom.size(); ++k) {
-        const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
-        mom[k] = float((t - 1.0) / tn);
-        t = tn;
-    }
-    w.mom.alloc(mom.size());
-    SF_HIP(hipMemcpyAsync(w.mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
-    L.mom = w.mom.p;
+    L.step = float(ops.step);
     if (L.path_n > 0) {
         w.Xpath.alloc(size_t(V) * L.path_n * d.Kp);
         w.itspath.alloc(size_t(V) * L.path_n);
@@ -313,7 +345,6 @@ void setup_work(FistaLaunch& L, FistaWork& w, const Design& d, const FistaOps& o
         L.itspath = w.itspath.p;
         L.convpath = w.convpath.p;
     }
-    SF_HIP(hipStreamSynchronize(s));  // the host momentum table goes out of scope
     if (L.precision == SHOREFIT_PREC_FP32) return;
     w.counter.alloc(1);
     w.X.alloc(size_t(V) * d.Kp);
@@ -468,10 +499,7 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         const bool mixed = precision != SHOREFIT_PREC_FP32;
         Design d;
         upload_design(d, A, n_rows, n_cols, s, mixed);
-        gram_and_lipschitz(d, s);
-        if (lipschitz_out) *lipschitz_out = d.L;
         FistaOps ops;
-        pack_fista(d, ops, precision, s);
         const int64_t V = n_signals;
         DevBuf<float> S(size_t(V) * n_rows), C(size_t(V) * n_cols), R(V);
         DevBuf<uint32_t> It(V);
@@ -487,10 +515,6 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         L.K = d.K;
         L.Kp = d.Kp;
         L.Np = d.Np;
-        L.Mfrag = ops.M.p;
-        L.Afrag = ops.A.p;
-        L.step = float(ops.step);
-        L.thr = float(ops.step * alpha);
         L.ratio = 0.f;
         L.alpha = alpha;
         L.max_iter = max_iter;
@@ -502,8 +526,9 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
         L.resid = R.p;
         L.iters = It.p;
         L.converged = Cv.p;
-        setup_work(L, work, d, ops, V, s);
-        launch_fista_rhs(L, s);
+        setup_fista(d, ops, work, L, s);
+        if (lipschitz_out) *lipschitz_out = d.L;
+        L.thr = float(ops.step * alpha);
         launch_fista(L, s);
         SF_HIP(hipStreamSynchronize(s));
         if (V) {
@@ -733,10 +758,7 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         SF_HIP(hipEventCreate(&e0));
         SF_HIP(hipEventCreate(&e1));
         if (l1) {
-            gram_and_lipschitz(d, s);
-            out->lipschitz = d.L;
             FistaOps ops;
-            pack_fista(d, ops, precision, s);
             FistaLaunch L{};
             FistaWork work;
             L.data = data;
@@ -747,10 +769,6 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
             L.K = d.K;
             L.Kp = d.Kp;
             L.Np = d.Np;
-            L.Mfrag = ops.M.p;
-            L.Afrag = ops.A.p;
-            L.step = float(ops.step);
-            L.thr = float(ops.step * sp.alpha);
             L.ratio = sp.alpha_mode == SHOREFIT_ALPHA_RATIO ? float(sp.alpha_ratio) : 0.f;
             L.alpha = sp.alpha;
             L.max_iter = sp.max_iter;
@@ -791,28 +809,29 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                 L.path_rule = anchored ? 1 : 0;
                 L.slack = sp.slack;
                 L.rss_l2_ref = rssl2;
-                if (anchored && V > 0) {
-                    // the Tikhonov reference fit (src/fit.rs:404-410, 760-761)
-                    std::vector<double> dn, dl;
-                    regularization_diagonals(sp.radial_order, dn, dl);
-                    pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, tik, s);
-                    TikhonovLaunch T{};
-                    T.data = data;
-                    T.vox = vox_dev;
-                    T.n_vox = V;
-                    T.N = d.N;
-                    T.sig_stride = N;
-                    T.K = d.K;
-                    T.Kp = d.Kp;
-                    T.Np = d.Np;
-                    T.Pfrag = tik.P.p;
-                    T.Afrag = tik.A.p;
-                    T.rss_l2 = rssl2;
-                    launch_tikhonov(T, s);
-                }
             }
-            setup_work(L, work, d, ops, V, s);
-            launch_fista_rhs(L, s);
+            setup_fista(d, ops, work, L, s);
+            out->lipschitz = d.L;
+            L.thr = float(ops.step * sp.alpha);
+            if (anchored && V > 0) {
+                // the Tikhonov reference fit (src/fit.rs:404-410, 760-761)
+                std::vector<double> dn, dl;
+                regularization_diagonals(sp.radial_order, dn, dl);
+                pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, tik, s);
+                TikhonovLaunch T{};
+                T.data = data;
+                T.vox = vox_dev;
+                T.n_vox = V;
+                T.N = d.N;
+                T.sig_stride = N;
+                T.K = d.K;
+                T.Kp = d.Kp;
+                T.Np = d.Np;
+                T.Pfrag = tik.P.p;
+                T.Afrag = tik.A.p;
+                T.rss_l2 = rssl2;
+                launch_tikhonov(T, s);
+            }
             SF_HIP(hipEventRecord(e0, s));
             L.event_mid = e1;  // recorded right after the solve kernel
             launch_fista(L, s);

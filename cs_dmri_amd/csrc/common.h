@@ -99,6 +99,12 @@ void launch_gram(const double* d_A, int N, int K, double* d_G, hipStream_t s);
 // Largest eigenvalue of the symmetric K x K matrix G by power iteration
 // (Rayleigh quotient), written to d_lambda[0].
 void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStream_t s);
+// From the largest eigenvalue lam of A^T A: out[0] = L = max(lam / N, eps)
+// (src/solver/fista.rs:51,58), out[1] = c = (1 / L) / N, on the device.
+void launch_fit_scalars(const double* d_lam, int N, double* d_out, hipStream_t s);
+// Synchronize s unless an entry point's StreamScope is active (whose buffers
+// are only recycled after the call's final synchronization).
+void sync_if_unscoped(hipStream_t s);
 // In-place lower Cholesky of Greg (K x K, upper triangle zeroed); d_status[0]
 // = 0 on success, j+1 when pivot j is not positive.
 void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s);
@@ -135,7 +141,7 @@ struct FistaLaunch {
     const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
     const uint16_t* Mlow2;  // FP16 remainder of -cG x 2^12 (split refresh, FP16 only)
     const double* A64p;     // FP64 design zero-padded to N x Kp (right-hand sides)
-    double c;               // step / N
+    const double* c_dev;    // step / N on the device (read by the right-hand-side kernel)
     float* Bbuf;            // scratch [n_vox][Kp]: b = c A^T s
     float* meanbuf;         // scratch [n_vox]: signal means
     float thr;              // step * alpha (fixed alpha)

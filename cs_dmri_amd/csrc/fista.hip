@@ -195,8 +195,8 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
 template <int KQ>  // KQ = Kp / 4 coefficients per wave
 __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data, const int64_t* __restrict__ vox,
                                                   int64_t n_vox, int64_t sig_stride, int N, int Kp,
-                                                  const double* __restrict__ Ap, double c, float* __restrict__ B,
-                                                  float* __restrict__ mean) {
+                                                  const double* __restrict__ Ap, const double* __restrict__ cp,
+                                                  float* __restrict__ B, float* __restrict__ mean) {
     constexpr int CH = 64;  // samples per LDS chunk
     __shared__ float sm[64][CH + 1];
     const int lane = threadIdx.x & 63;
@@ -227,6 +227,7 @@ __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data
     }
     const int64_t v = v0 + lane;
     if (v >= n_vox) return;
+    const double c = *cp;  // step / N, computed on the device from the Lipschitz constant
 #pragma unroll
     for (int q = 0; q < KQ; q += 4)
         *reinterpret_cast<floatx4*>(B + v * Kp + k0 + q) =
@@ -443,7 +444,7 @@ template <int MB>
 void launch_rhs(const FistaLaunch& L, hipStream_t s) {
     const unsigned grid = unsigned((L.n_vox + 63) / 64);
     hipLaunchKernelGGL(rhs_kernel<4 * MB>, dim3(grid), dim3(256), 0, s, L.data, L.vox, L.n_vox, L.sig_stride, L.N,
-                       L.Kp, L.A64p, L.c, L.Bbuf, L.meanbuf);
+                       L.Kp, L.A64p, L.c_dev, L.Bbuf, L.meanbuf);
     SF_HIP(hipGetLastError());
 }
 

@@ -430,6 +430,12 @@ __global__ void __launch_bounds__(256) mask_write_kernel(const uint8_t* __restri
     }
 }
 
+__global__ void fit_scalars_kernel(const double* __restrict__ lam, int N, double* __restrict__ out) {
+    const double L = fmax(fmax(*lam, 0.0) / double(N), 2.220446049250313e-16);
+    out[0] = L;
+    out[1] = (1.0 / L) / double(N);
+}
+
 __global__ void gather_f32_kernel(const float* __restrict__ src, const int64_t* __restrict__ idx, int64_t n,
                                   float* __restrict__ dst) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -490,7 +496,16 @@ void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStrea
     hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(1024), size_t(2 * K) * sizeof(double), s, src, d_G, K,
                        d_lambda);
     SF_HIP(hipGetLastError());
-    SF_HIP(hipStreamSynchronize(s));  // p0 / p1 are released at scope exit
+    sync_if_unscoped(s);  // p0 / p1 are released at scope exit
+}
+
+void launch_fit_scalars(const double* d_lam, int N, double* d_out, hipStream_t s) {
+    hipLaunchKernelGGL(fit_scalars_kernel, dim3(1), dim3(1), 0, s, d_lam, N, d_out);
+    SF_HIP(hipGetLastError());
+}
+
+void sync_if_unscoped(hipStream_t s) {
+    if (!stream_scope_active()) SF_HIP(hipStreamSynchronize(s));
 }
 
 void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s) {
