@@ -61,6 +61,7 @@ def parse():
     p.add_argument("--refresh", type=int, default=10)
     p.add_argument("--cpu-sample", type=int, default=32768)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-postfit", action="store_true", help="skip the post-fit map timings")
     p.add_argument("--max-iter", type=int, default=500)
     p.add_argument("--alpha-mode", default="fixed", choices=["fixed", "alpha-ratio", "path-bic", "l2-anchored"],
                    help="alpha strategy (the headline line is the fixed-alpha fit)")
@@ -80,6 +81,45 @@ def copy_calibration(mib: int):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches": 3}}), flush=True)
+
+
+def postfit_timings(L, d_coef, d_mask, wl, K, N, dev):
+    """Post-fit maps on the fitted volume, device-resident: the Tournier SH
+    projection (lmax 8), synthesis on the fit's own table and the six
+    microstructure maps (with fit-failure rejection). Mean of 3 calls after
+    one warm-up, each a whole entry-point call (set-up, kernels, stream sync).
+    These maps are HBM-bound (a K-vector in, a few floats out per voxel)."""
+    import torch
+    nx, ny, nz = wl.mask.shape
+    nv = nx * ny * nz
+    g = wl.gtab
+    sh = torch.empty((nx, ny, nz, 45), dtype=torch.float32, device=dev)
+    pred = torch.empty((nx, ny, nz, N), dtype=torch.float32, device=dev)
+    maps = [torch.empty((nx, ny, nz), dtype=torch.float32, device=dev) for _ in range(6)]
+    grads, _ = L.gradients(g.bvals, g.bvecs, g.big_delta, g.small_delta, g.b0_threshold)
+    calls = {
+        "odf_sh": (lambda: L.lib().shorefit_odf_sh(d_coef.data_ptr(), d_mask.data_ptr(), nx, ny, nz, 8, wl.zeta, 8,
+                                                   sh.data_ptr(), 1, None),
+                   wl.n_voxels * 4 * (K + 45)),
+        "predict": (lambda: L.lib().shorefit_predict(d_coef.data_ptr(), nx, ny, nz, 8, wl.zeta, C.byref(grads),
+                                                     pred.data_ptr(), 1, None),
+                    nv * 4 * (K + N)),
+        "microstructure": (lambda: L.lib().shorefit_microstructure(d_coef.data_ptr(), d_mask.data_ptr(), nx, ny, nz,
+                                                                   8, wl.zeta, None, L.UNITS_UM, 10.0,
+                                                                   *[m.data_ptr() for m in maps], 1, None),
+                           wl.n_voxels * 4 * (K + 6)),
+    }
+    out = {}
+    for name, (fn, nbytes) in calls.items():
+        L.check(fn())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            L.check(fn())
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / 3 * 1e3
+        out[name] = {"ms": round(ms, 3), "algorithmic_bytes": nbytes, "GB_s": round(nbytes / ms / 1e6, 1)}
+    return out
 
 
 def main():
@@ -168,6 +208,7 @@ def main():
                              "tflops": round(executed / kern_s / 1e12, 3),
                              "frac": round(executed / kern_s / 1e12 / peak, 4)}}
 
+    postfit = postfit_timings(L, d_coef, d_mask, wl, K, N, dev) if not args.no_postfit else None
     coef = d_coef.cpu().numpy()[wl.mask]
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "voxel-fits/s", "n_gpus": 1,
@@ -221,6 +262,8 @@ def main():
                               "coef_rel_err_max": float(err.max()),
                               "frac_le_1e-4": float(np.mean(err <= 1e-4)),
                               "iterations_equal_frac": float(np.mean(ref["iterations"].ravel() == its[pick]))}}
+    if postfit is not None:
+        result["postfit"] = postfit
     print(json.dumps(result), flush=True)
 
 

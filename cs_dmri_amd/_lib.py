@@ -19,9 +19,12 @@ REG = {"l1": 1, "l2": 2, "nonneg": 3}
 ALPHA_MODE = {"fixed": 0, "alpha-ratio": 1, "path-bic": 2, "l2-anchored": 3}
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
 
+UNITS_UM, UNITS_MM = 0, 1
+
 EXPORTS = (
     "shorefit_spec_default", "shorefit_fit", "shorefit_n_coeffs", "shorefit_design_matrix",
     "shorefit_fista_batch", "shorefit_tikhonov_batch", "shorefit_last_error", "shorefit_version",
+    "shorefit_tournier_matrix", "shorefit_odf_sh", "shorefit_predict", "shorefit_microstructure",
 )
 
 
@@ -99,6 +102,18 @@ def _load() -> C.CDLL:
     lib.shorefit_last_error.restype = C.c_char_p
     lib.shorefit_version.argtypes = []
     lib.shorefit_version.restype = C.c_char_p
+    _vp = C.c_void_p
+    lib.shorefit_tournier_matrix.argtypes = [C.c_uint32, C.c_double, C.c_uint32, _pd]
+    lib.shorefit_tournier_matrix.restype = C.c_int
+    lib.shorefit_odf_sh.argtypes = [_vp, _vp, C.c_int64, C.c_int64, C.c_int64, C.c_uint32, C.c_double, C.c_uint32,
+                                    _vp, C.c_int32, _vp]
+    lib.shorefit_odf_sh.restype = C.c_int
+    lib.shorefit_predict.argtypes = [_vp, C.c_int64, C.c_int64, C.c_int64, C.c_uint32, C.c_double,
+                                     C.POINTER(ShorefitGradients), _vp, C.c_int32, _vp]
+    lib.shorefit_predict.restype = C.c_int
+    lib.shorefit_microstructure.argtypes = [_vp, _vp, C.c_int64, C.c_int64, C.c_int64, C.c_uint32, C.c_double, _vp,
+                                            C.c_int32, C.c_float] + [_vp] * 6 + [C.c_int32, _vp]
+    lib.shorefit_microstructure.restype = C.c_int
     return lib
 
 

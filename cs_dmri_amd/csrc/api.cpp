@@ -439,6 +439,27 @@ void regularization_diagonals(uint32_t order, std::vector<double>& dn, std::vect
 
 using namespace shorefit;
 
+namespace {
+// Device copy of a host array (or the device pointer itself).
+template <typename T>
+const T* dev_in(const T* p, size_t count, bool on_device, DevBuf<T>& buf, hipStream_t s) {
+    if (!p || on_device) return p;
+    buf.alloc(count);
+    if (count) SF_HIP(hipMemcpyAsync(buf.p, p, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    return buf.p;
+}
+
+// Masked-voxel list on the device (null = every voxel); returns the count.
+int64_t voxel_list(const uint8_t* d_mask, int64_t n, DevBuf<int64_t>& vox, hipStream_t s) {
+    if (!d_mask) return n;
+    return compact_mask(d_mask, n, vox, s);
+}
+
+void check_dims(int64_t nx, int64_t ny, int64_t nz) {
+    if (nx < 0 || ny < 0 || nz < 0) throw Error{SHOREFIT_E_INVALID, "negative volume dimension"};
+}
+}  // namespace
+
 extern "C" {
 
 const char* shorefit_last_error(void) { return g_last_error.c_str(); }
@@ -914,6 +935,148 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
                 SF_HIP(hipMemcpy(out->rss_l2, rssl2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
         }
         warn.to(out);
+    });
+}
+
+// --------------------------------------------------------- post-fit maps ----
+
+int shorefit_tournier_matrix(uint32_t radial_order, double zeta, uint32_t lmax, double* out) {
+    return guarded([&] {
+        if (!out) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        if (lmax % 2 != 0 || lmax > radial_order)
+            throw Error{SHOREFIT_E_INVALID, "lmax must be even and ≤ radial_order (" + std::to_string(radial_order) +
+                                                "), got " + std::to_string(lmax)};
+        tournier_matrix_host(radial_order, zeta, lmax, out);
+    });
+}
+
+
+int shorefit_odf_sh(const float* coefficients, const uint8_t* mask, int64_t nx, int64_t ny, int64_t nz,
+                    uint32_t radial_order, double zeta, uint32_t lmax, float* out, int32_t on_device, void* stream) {
+    return guarded([&] {
+        if (!coefficients || !out) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        check_dims(nx, ny, nz);
+        if (lmax % 2 != 0 || lmax > radial_order)
+            throw Error{SHOREFIT_E_INVALID, "lmax must be even and ≤ radial_order (" + std::to_string(radial_order) +
+                                                "), got " + std::to_string(lmax)};
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        const int K = n_coeffs(radial_order), R = n_tournier(lmax);
+        const int64_t nv = nx * ny * nz;
+        std::vector<double> M(size_t(R) * K);
+        tournier_matrix_host(radial_order, zeta, lmax, M.data());
+        DevBuf<double> dM(M.size());
+        SF_HIP(hipMemcpyAsync(dM.p, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, s));
+        const bool dev = on_device != 0;
+        DevBuf<float> bc, bo;
+        DevBuf<uint8_t> bm;
+        const float* c = dev_in(coefficients, size_t(nv) * K, dev, bc, s);
+        const uint8_t* m = dev_in(mask, size_t(nv), dev, bm, s);
+        float* o = out;
+        if (!dev) {
+            bo.alloc(size_t(nv) * R);
+            o = bo.p;
+        }
+        if (nv) SF_HIP(hipMemsetAsync(o, 0, sizeof(float) * size_t(nv) * R, s));
+        DevBuf<int64_t> vox;
+        const int64_t V = voxel_list(m, nv, vox, s);
+        launch_apply_rows(c, vox.p, V, K, dM.p, R, false, o, s);
+        if (!dev && nv) SF_HIP(hipMemcpyAsync(out, o, sizeof(float) * size_t(nv) * R, hipMemcpyDeviceToHost, s));
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int shorefit_predict(const float* coefficients, int64_t nx, int64_t ny, int64_t nz, uint32_t radial_order,
+                     double zeta, const ShorefitGradients* target, float* out, int32_t on_device, void* stream) {
+    return guarded([&] {
+        if (!coefficients || !out || !target) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        check_dims(nx, ny, nz);
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        double big = target->big_delta, small = target->small_delta;
+        if (std::isnan(big) || std::isnan(small)) tortoise_deltas(target->bvals, target->n_grads, &big, &small);
+        Design d;
+        build_design(d, *target, big, small, radial_order, zeta, s);
+        const int K = d.K, Nt = d.N;
+        const int64_t nv = nx * ny * nz;
+        const bool dev = on_device != 0;
+        DevBuf<float> bc, bo;
+        const float* c = dev_in(coefficients, size_t(nv) * K, dev, bc, s);
+        float* o = out;
+        if (!dev) {
+            bo.alloc(size_t(nv) * Nt);
+            o = bo.p;
+        }
+        launch_apply_rows(c, nullptr, nv, K, d.A.p, Nt, true, o, s);
+        if (!dev && nv) SF_HIP(hipMemcpyAsync(out, o, sizeof(float) * size_t(nv) * Nt, hipMemcpyDeviceToHost, s));
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int shorefit_microstructure(const float* coefficients, const uint8_t* mask, int64_t nx, int64_t ny, int64_t nz,
+                            uint32_t radial_order, double zeta, const float* directions, int32_t units,
+                            float outlier_factor, float* rtop, float* rtap, float* rtpp, float* msd, float* qiv,
+                            float* ng, int32_t on_device, void* stream) {
+    return guarded([&] {
+        if (!coefficients || !mask) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        check_dims(nx, ny, nz);
+        if (units != SHOREFIT_UNITS_UM && units != SHOREFIT_UNITS_MM)
+            throw Error{SHOREFIT_E_INVALID, "units must be 'um' or 'mm'"};
+        if (radial_order > 14) throw Error{SHOREFIT_E_UNSUPPORTED, "radial order above 14"};
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        const int K = n_coeffs(radial_order);
+        const int64_t nv = nx * ny * nz;
+        const bool dev = on_device != 0;
+        MicroTables T;
+        micro_tables_host(radial_order, zeta, units, T);
+        DevBuf<float> bc, bd;
+        DevBuf<uint8_t> bm;
+        const float* c = dev_in(coefficients, size_t(nv) * K, dev, bc, s);
+        const uint8_t* m = dev_in(mask, size_t(nv), dev, bm, s);
+        const float* dirs = dev_in(directions, size_t(nv) * 3, dev, bd, s);
+        DevBuf<int64_t> vox;
+        const int64_t V = voxel_list(m, nv, vox, s);
+        DevBuf<float> raw(size_t(6) * std::max<int64_t>(V, 1));
+        launch_micro(c, vox.p, V, K, T, dirs, raw.p, s);
+        // src/io/microstructure.rs:215-268, 303-326: threshold = factor x p99
+        // of the finite values (index round((n - 1) 0.99) of the sorted list)
+        MicroThresholds th{};
+        if (outlier_factor > 0.f && V > 0) {
+            std::vector<float> h(size_t(6) * V);
+            SF_HIP(hipMemcpyAsync(h.data(), raw.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, s));
+            SF_HIP(hipStreamSynchronize(s));
+            for (int q = 0; q < 6; ++q) {
+                std::vector<float> f;
+                f.reserve(V);
+                for (int64_t v = 0; v < V; ++v) {
+                    const float x = h[size_t(q) * V + v];
+                    if (std::isfinite(x)) f.push_back(x);
+                }
+                if (f.empty()) continue;
+                const size_t i = std::min(size_t(std::llround(0.99 * double(f.size() - 1))), f.size() - 1);
+                std::nth_element(f.begin(), f.begin() + i, f.end());
+                th.active[q] = 1;
+                th.thr[q] = outlier_factor * f[i];
+            }
+        }
+        float* user[6] = {rtop, rtap, rtpp, msd, qiv, ng};
+        DevBuf<float> bo[6];
+        float* maps[6];
+        for (int q = 0; q < 6; ++q) {
+            maps[q] = user[q];
+            if (user[q] && !dev) {
+                bo[q].alloc(std::max<int64_t>(nv, 1));
+                maps[q] = bo[q].p;
+            }
+            if (maps[q] && nv) SF_HIP(hipMemsetAsync(maps[q], 0, sizeof(float) * nv, s));
+        }
+        launch_micro_scatter(raw.p, vox.p, V, th, maps, s);
+        if (!dev)
+            for (int q = 0; q < 6; ++q)
+                if (user[q] && nv)
+                    SF_HIP(hipMemcpyAsync(user[q], maps[q], sizeof(float) * nv, hipMemcpyDeviceToHost, s));
+        SF_HIP(hipStreamSynchronize(s));
     });
 }
 

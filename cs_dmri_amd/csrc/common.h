@@ -206,4 +206,33 @@ struct TikhonovLaunch {
 };
 void launch_tikhonov(const TikhonovLaunch& L, hipStream_t s);
 
+// ------------------------------------------------------------- post-fit ----
+// Per-mode weights of the microstructure scalars for one (radial order,
+// zeta); blocks are the (n, l) pairs in coefficient order.
+struct MicroTables {
+    int R;
+    int n00[16];                      // column of (n, 0, 0)
+    double e0w[16], rtopw[16], msdw[16], qivw[16];
+    int nblocks;
+    int blk_l[64], blk_start[64];
+    double scale_ap[64], scale_pp[64];  // RTAP / RTPP kernel over alpha(n, l)
+    double qiv_pre, pref_ap, pref_pp;
+    float unit[6];                    // rtop, rtap, rtpp, msd, qiv, ng
+};
+struct MicroThresholds {
+    int active[6];
+    float thr[6];
+};
+int n_tournier(unsigned lmax);
+void tournier_matrix_host(unsigned radial_order, double zeta, unsigned lmax, double* out);
+void micro_tables_host(unsigned radial_order, double zeta, int units, MicroTables& T);
+// out[vox[v] * R + r] = sum_k M[r][k] coef[vox[v] * K + k] (FP64); with
+// skip_zero an all-zero coefficient vector gives zeros.
+void launch_apply_rows(const float* coef, const int64_t* vox, int64_t V, int K, const double* M, int R, bool skip_zero,
+                       float* out, hipStream_t s);
+void launch_micro(const float* coef, const int64_t* vox, int64_t V, int K, const MicroTables& T, const float* dirs,
+                  float* raw, hipStream_t s);
+void launch_micro_scatter(const float* raw, const int64_t* vox, int64_t V, const MicroThresholds& th, float* const* maps,
+                          hipStream_t s);
+
 }  // namespace shorefit

@@ -214,3 +214,78 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
         if rss_l2 is not None:
             result["rss_l2"] = rss_l2
     return result
+
+
+def _coef_volume(coefficients, radial_order):
+    c = np.ascontiguousarray(np.asarray(coefficients), dtype=np.float32)
+    if c.ndim != 4:
+        raise ValueError(f"coefficients must be 4-D (X, Y, Z, K), got shape {c.shape}")
+    K = n_coeffs(radial_order)
+    if c.shape[3] != K:
+        raise ValueError(f"coefficients have {c.shape[3]} channels but a radial-order-{radial_order} SHORE basis "
+                         f"has {K}")
+    return c
+
+
+def tournier_matrix(radial_order, zeta, lmax) -> np.ndarray:
+    """SHORE -> Tournier SH projection (src/odf.rs:79-102), (n_sh, K) float64."""
+    K = n_coeffs(radial_order)
+    M = np.zeros(((lmax + 1) * (lmax + 2) // 2, K))
+    if lmax % 2 != 0 or lmax > radial_order:
+        raise ValueError(f"lmax must be even and ≤ radial_order ({radial_order}), got {lmax}")
+    L.check(L.lib().shorefit_tournier_matrix(radial_order, zeta, lmax, L.ptr(M, L._pd)))
+    return M
+
+
+def shore_odf_sh(coefficients, radial_order, zeta, lmax, mask=None, *, n_threads=None) -> np.ndarray:
+    """Drop-in for ``_cs_dmri.shore_odf_sh`` (python/src/models.rs:203-248)."""
+    if lmax % 2 != 0 or lmax > radial_order:
+        raise ValueError(f"lmax must be even and ≤ radial_order ({radial_order}), got {lmax}")
+    c = _coef_volume(coefficients, radial_order)
+    nx, ny, nz, _ = c.shape
+    m = None if mask is None else np.ascontiguousarray(np.asarray(mask), dtype=np.uint8)
+    if m is not None and m.shape != (nx, ny, nz):
+        raise ValueError("mask does not match the coefficient grid")
+    out = np.zeros((nx, ny, nz, (lmax + 1) * (lmax + 2) // 2), np.float32)
+    L.check(L.lib().shorefit_odf_sh(c.ctypes.data, None if m is None else m.ctypes.data, nx, ny, nz, radial_order,
+                                    zeta, lmax, out.ctypes.data, 0, None))
+    return out
+
+
+def shore_predict(coefficients, radial_order, zeta, bvals, bvecs, *, big_delta=None, small_delta=None,
+                  n_threads=None) -> np.ndarray:
+    """Drop-in for ``_cs_dmri.shore_predict`` (python/src/models.rs:255-278)."""
+    c = _coef_volume(coefficients, radial_order)
+    nx, ny, nz, _ = c.shape
+    g, _keep = L.gradients(bvals, bvecs, big_delta, small_delta, 50.0)
+    out = np.zeros((nx, ny, nz, g.n_grads), np.float32)
+    L.check(L.lib().shorefit_predict(c.ctypes.data, nx, ny, nz, radial_order, zeta, C.byref(g), out.ctypes.data, 0,
+                                     None))
+    return out
+
+
+def shore_microstructure(coefficients, radial_order, zeta, mask, directions=None, *, units="um",
+                         outlier_factor=10.0, n_threads=None) -> dict:
+    """Drop-in for ``_cs_dmri.shore_microstructure`` (python/src/models.rs:286-331): RTOP, RTAP, RTPP,
+    MSD, QIV and NG maps (zero outside ``mask``; NaN for rejected fits and, for RTAP / RTPP, missing
+    directions)."""
+    c = _coef_volume(coefficients, radial_order)
+    nx, ny, nz, _ = c.shape
+    m = np.ascontiguousarray(np.asarray(mask), dtype=np.uint8)
+    if m.shape != (nx, ny, nz):
+        raise ValueError("mask does not match the coefficient grid")
+    if units not in ("um", "mm"):
+        raise ValueError(f"units must be 'um' or 'mm', got {units!r}")
+    d = None
+    if directions is not None:
+        d = np.ascontiguousarray(np.asarray(directions), dtype=np.float32)
+        if d.shape != (nx, ny, nz, 3):
+            raise ValueError(f"directions must have shape {(nx, ny, nz, 3)}, got {d.shape}")
+    names = ("rtop", "rtap", "rtpp", "msd", "qiv", "ng")
+    maps = {k: np.zeros((nx, ny, nz), np.float32) for k in names}
+    L.check(L.lib().shorefit_microstructure(c.ctypes.data, m.ctypes.data, nx, ny, nz, radial_order, zeta,
+                                            None if d is None else d.ctypes.data,
+                                            {"um": L.UNITS_UM, "mm": L.UNITS_MM}[units],
+                                            -1.0 if outlier_factor is None else float(outlier_factor),
+                                            *[maps[k].ctypes.data for k in names], 0, None))
+    return maps

@@ -163,6 +163,47 @@ int shorefit_tikhonov_batch(const double* A, int32_t n_rows, int32_t n_cols, con
                             double lambda_n, double lambda_l, const float* signals, int64_t n_signals, float* coef,
                             float* residual_l2);
 
+/* ---------------------------------------------------------- post-fit maps ----
+ * Coefficient volumes are (X,Y,Z,K) float32 in C order, as shorefit_fit
+ * writes them. Output volumes are zeroed outside the mask. With on_device the
+ * pointers are device memory; otherwise host memory (copied in and out). */
+
+#define SHOREFIT_UNITS_UM 0 /* q in 1/um: RTOP /um^3, RTAP /um^2, RTPP /um, MSD um^2, QIV um^5 */
+#define SHOREFIT_UNITS_MM 1 /* q in 1/mm (the fit's own units) */
+
+/* SHORE -> Tournier (MRtrix) SH projection, row-major (lmax+1)(lmax+2)/2 x K
+ * (FP64, host memory); lmax even and <= radial_order. Replaces
+ * cs_dmri::odf::shore_to_tournier_sh_matrix (src/odf.rs:79-102). */
+int shorefit_tournier_matrix(uint32_t radial_order, double zeta, uint32_t lmax, double* out);
+
+/* ODF in Tournier SH per voxel: out (X,Y,Z,(lmax+1)(lmax+2)/2) float32 =
+ * projection x coefficients (FP64 sums), zeros outside mask (mask may be
+ * NULL = every voxel). Replaces _cs_dmri.shore_odf_sh
+ * (python/src/models.rs:203-248). */
+int shorefit_odf_sh(const float* coefficients, const uint8_t* mask, int64_t nx, int64_t ny, int64_t nz,
+                    uint32_t radial_order, double zeta, uint32_t lmax, float* out, int32_t on_device, void* stream);
+
+/* Signal synthesized for a target gradient table: out (X,Y,Z,N') float32 =
+ * design(target) x coefficients (FP64 sums); voxels whose coefficients are
+ * all zero give zeros. Replaces cs_dmri::synthesize_volume
+ * (src/synth.rs:15-84) behind _cs_dmri.shore_predict
+ * (python/src/models.rs:255-278). */
+int shorefit_predict(const float* coefficients, int64_t nx, int64_t ny, int64_t nz, uint32_t radial_order,
+                     double zeta, const ShorefitGradients* target, float* out, int32_t on_device, void* stream);
+
+/* RTOP, RTAP, RTPP, MSD, QIV and NG maps over the mask (required), in the
+ * given units; directions (X,Y,Z,3) float32 (may be NULL) are the peak
+ * directions for RTAP / RTPP (zero or non-finite = no peak -> NaN). When
+ * outlier_factor > 0, each map's finite values above outlier_factor x its
+ * 99th percentile become NaN. Any output pointer may be NULL. Replaces
+ * _cs_dmri.shore_microstructure (python/src/models.rs:286-331) /
+ * cs_dmri::io::microstructure::compute_microstructure
+ * (src/io/microstructure.rs:160-300). */
+int shorefit_microstructure(const float* coefficients, const uint8_t* mask, int64_t nx, int64_t ny, int64_t nz,
+                            uint32_t radial_order, double zeta, const float* directions, int32_t units,
+                            float outlier_factor, float* rtop, float* rtap, float* rtpp, float* msd, float* qiv,
+                            float* ng, int32_t on_device, void* stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* shorefit_last_error(void);
 

@@ -73,6 +73,16 @@ class Oracle:
                                     C.c_uint32, _d, _i, _pd, _pd, _d, _d, _i, _pf, _pf, _pf, _pu32, _pu8,
                                     _pf, _pf, _pf, _pd]),
             "orc_effective_table": (None, [_pd, _pd, _pd, _pd, _pd, _i, _d, _pd, _pd]),
+            "orc_hyp2f1": (_d, [_d, _d, _d, _d]),
+            "orc_binomial_float": (_d, [_d, _u]),
+            "orc_odf_radial_factor": (_d, [_u, _u, _d]),
+            "orc_brainsuite_to_tournier": (None, [_u, _i, C.POINTER(C.c_int), _pd]),
+            "orc_tournier_matrix": (None, [_u, _d, _u, _pd]),
+            "orc_alpha_factor": (_d, [_u, _u, _d]),
+            "orc_microstructure_raw": (None, [_u, _d, _pd, _pd, _pd]),
+            "orc_compute_microstructure": (None, [_u, _d, _pf, _pf, C.c_int64, _i, C.c_float, _pf]),
+            "orc_synthesize": (None, [_pd, _i, _i, _pf, C.c_int64, _pf]),
+            "orc_project": (None, [_pd, _i, _i, _pf, C.c_int64, _pf]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -243,6 +253,64 @@ class Oracle:
         self.lib.orc_effective_table(_ptr(T, _pd), _ptr(R, _pd), _ptr(b, _pd), _ptr(gv, _pd), _ptr(gn, _pd),
                                      len(b), b0_threshold, _ptr(bo, _pd), _ptr(do, _pd))
         return bo, do
+
+    # -- post-fit (src/odf.rs, src/scalars/microstructure.rs, src/synth.rs) --
+    def hyp2f1(self, a, b, c, z) -> float:
+        return self.lib.orc_hyp2f1(a, b, c, z)
+
+    def binomial_float(self, n, k) -> float:
+        return self.lib.orc_binomial_float(n, k)
+
+    def odf_radial_factor(self, n, ell, zeta) -> float:
+        return self.lib.orc_odf_radial_factor(n, ell, zeta)
+
+    def brainsuite_to_tournier(self, ell, k):
+        row, sign = C.c_int(0), C.c_double(0)
+        self.lib.orc_brainsuite_to_tournier(ell, k, C.byref(row), C.byref(sign))
+        return row.value, sign.value
+
+    def tournier_matrix(self, radial_order, zeta, lmax) -> np.ndarray:
+        K = self.n_coeffs(radial_order)
+        M = np.zeros(((lmax + 1) * (lmax + 2) // 2, K))
+        self.lib.orc_tournier_matrix(radial_order, zeta, lmax, _ptr(M, _pd))
+        return M
+
+    def alpha_factor(self, n, ell, zeta) -> float:
+        return self.lib.orc_alpha_factor(n, ell, zeta)
+
+    def microstructure_raw(self, radial_order, zeta, coefs, direction=None) -> np.ndarray:
+        """rtop, rtap, rtpp, msd, qiv, ng (FP64, mm units) of one coefficient vector."""
+        c = _f64(coefs)
+        d = None if direction is None else _f64(direction)
+        out = np.zeros(6)
+        self.lib.orc_microstructure_raw(radial_order, zeta, _ptr(c, _pd), _ptr(d, _pd), _ptr(out, _pd))
+        return out
+
+    def compute_microstructure(self, radial_order, zeta, coefs, dirs=None, units="um", outlier_factor=10.0):
+        """Per masked voxel (rows of coefs), dict of the six f32 scalars."""
+        c = np.ascontiguousarray(coefs, dtype=np.float32)
+        V = c.shape[0]
+        d = None if dirs is None else np.ascontiguousarray(dirs, dtype=np.float32)
+        out = np.zeros((6, V), np.float32)
+        self.lib.orc_compute_microstructure(radial_order, zeta, _ptr(c, _pf), _ptr(d, _pf), V,
+                                            {"um": 0, "mm": 1}[units],
+                                            -1.0 if outlier_factor is None else float(outlier_factor),
+                                            _ptr(out, _pf))
+        return dict(zip(("rtop", "rtap", "rtpp", "msd", "qiv", "ng"), out))
+
+    def synthesize(self, A, coefs) -> np.ndarray:
+        A = _f64(A)
+        c = np.ascontiguousarray(coefs, dtype=np.float32)
+        out = np.zeros((c.shape[0], A.shape[0]), np.float32)
+        self.lib.orc_synthesize(_ptr(A, _pd), A.shape[0], A.shape[1], _ptr(c, _pf), c.shape[0], _ptr(out, _pf))
+        return out
+
+    def project(self, M, coefs) -> np.ndarray:
+        M = _f64(M)
+        c = np.ascontiguousarray(coefs, dtype=np.float32)
+        out = np.zeros((c.shape[0], M.shape[0]), np.float32)
+        self.lib.orc_project(_ptr(M, _pd), M.shape[0], M.shape[1], _ptr(c, _pf), c.shape[0], _ptr(out, _pf))
+        return out
 
 
 _SINGLETON: Oracle | None = None

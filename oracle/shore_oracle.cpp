@@ -21,6 +21,12 @@
 //   volume driver ........ per-voxel equality with the single-voxel entry
 //                          points, R^2 closed form, scatter layout
 //   graddev table ........ src/graddev.rs:218-253 KATs
+//   post-fit ............. src/odf.rs KATs (scipy radial factors, index
+//                          examples, projection vs direct SH evaluation),
+//                          scipy hyp2f1 / binom, quadrature of the SHORE
+//                          signal for RTOP, QIV, RTAP, RTPP and a finite-
+//                          difference Laplacian for MSD; NG closed cases;
+//                          src/io/microstructure.rs rejection KATs
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -540,6 +546,254 @@ VoxelResult resolve(const Problem& P, const double* s, double* coef) {
     return out;
 }
 
+
+// ------------------------------------------------------ post-fit (config 3) ----
+
+// src/math.rs:130-145 — Gauss 2F1 by its series (terminating for a = -n).
+double hyp2f1(double a, double b, double c, double z) {
+    double sum = 1.0, term = 1.0;
+    for (int k = 0; k < 200; ++k) {
+        const double kf = double(k);
+        term *= (a + kf) * (b + kf) / ((c + kf) * (kf + 1.0)) * z;
+        sum += term;
+        if (term == 0.0) break;
+        if (std::fabs(term) < 1e-15 * std::fmax(std::fabs(sum), 1.0)) break;
+    }
+    return sum;
+}
+
+// src/math.rs:150-152
+double binomial_float(double n, unsigned k) { return gamma_fn(n + 1.0) / (factorial(k) * gamma_fn(n - double(k) + 1.0)); }
+
+// src/odf.rs:35-50 — C(n,l) G(n,l) F(n,l).
+double odf_radial_factor(unsigned n, unsigned ell, double zeta) {
+    const double nf = double(n), lf = double(ell);
+    const double sign = ((int(n) - int(ell) / 2) % 2 == 0) ? 1.0 : -1.0;
+    const double c_nl = sign * std::sqrt(2.0 * factorial(n - ell) / (std::pow(zeta, 1.5) * gamma_fn(nf + 1.5)));
+    const double g_nl = (gamma_fn(lf / 2.0 + 1.5) * gamma_fn(nf + 1.5)) / (gamma_fn(lf + 1.5) * factorial(n - ell)) *
+                        std::pow(0.5, -lf / 2.0 - 1.5);
+    const double f_nl = hyp2f1(-nf + lf, lf / 2.0 + 1.5, lf + 1.5, 2.0);
+    return c_nl * g_nl * f_nl;
+}
+
+// src/odf.rs:58-77 — Tournier index and sign of BrainSuite slot k of block ell.
+void brainsuite_to_tournier(unsigned ell, int k, int* row, double* sign) {
+    const int base = int(ell * (ell + 1) / 2), kl = int(ell);
+    if (k < kl) {
+        *row = base + (kl - k);
+        *sign = 1.0;
+    } else if (k == kl) {
+        *row = base;
+        *sign = 1.0;
+    } else {
+        const int j = k - kl;
+        *row = base - j;
+        *sign = (j % 2 == 1) ? 1.0 : -1.0;
+    }
+}
+
+// src/odf.rs:79-102 — row-major (lmax+1)(lmax+2)/2 x K.
+void tournier_matrix(unsigned radial_order, double zeta, unsigned lmax, double* out) {
+    const int rows = int((lmax + 1) * (lmax + 2) / 2);
+    const int K = int(shore_indices(radial_order).size());
+    std::fill(out, out + size_t(rows) * K, 0.0);
+    int col = 0;
+    for (unsigned n = 0; n <= radial_order; ++n)
+        for (unsigned ell = 0; ell <= n; ell += 2) {
+            const int bs = int(2 * ell + 1);
+            if (ell <= lmax) {
+                const double radial = odf_radial_factor(n, ell, zeta);
+                for (int k = 0; k < bs; ++k) {
+                    int row;
+                    double sign;
+                    brainsuite_to_tournier(ell, k, &row, &sign);
+                    out[size_t(row) * K + col + k] = radial * sign;
+                }
+            }
+            col += bs;
+        }
+}
+
+// src/scalars/microstructure.rs:41-71 — (n, ell) blocks and the (n,0,0) columns.
+struct NlBlock {
+    unsigned n, ell;
+    int start, len;
+};
+std::vector<NlBlock> nl_blocks(unsigned radial_order) {
+    std::vector<NlBlock> b;
+    int start = 0;
+    for (unsigned n = 0; n <= radial_order; ++n)
+        for (unsigned ell = 0; ell <= n; ell += 2) {
+            b.push_back({n, ell, start, int(2 * ell + 1)});
+            start += int(2 * ell + 1);
+        }
+    return b;
+}
+int n00_index(const std::vector<NlBlock>& b, unsigned n) {
+    for (const auto& k : b)
+        if (k.n == n && k.ell == 0) return k.start;
+    return -1;
+}
+
+// src/scalars/microstructure.rs:79-87
+double predict_e0(const double* c, unsigned R, double zeta, const std::vector<NlBlock>& b) {
+    double e0 = 0.0;
+    const double inv = 1.0 / std::sqrt(4.0 * kPi);
+    for (unsigned n = 0; n <= R; ++n) {
+        const double r0 = kappa(zeta, n, 0) * gen_laguerre(n, 0.5, 0.0);
+        e0 += c[n00_index(b, n)] * r0 * inv;
+    }
+    return e0;
+}
+
+const double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+// src/scalars/microstructure.rs:96-113
+double rtop(const double* c, unsigned R, double zeta, const std::vector<NlBlock>& b) {
+    const double e0 = predict_e0(c, R, zeta, b);
+    if (!std::isfinite(e0) || e0 == 0.0) return kNaN;
+    double acc = 0.0;
+    for (unsigned n = 0; n <= R; ++n) {
+        const double cn = c[n00_index(b, n)] / e0;
+        const double sign = (n % 2 == 0) ? 1.0 : -1.0;
+        const double w = std::sqrt(16.0 * kPi * std::pow(zeta, 1.5) * gamma_fn(double(n) + 1.5) / factorial(n));
+        acc += cn * sign * w;
+    }
+    return std::fmax(acc, 0.0);  // f64::max: a NaN operand yields the other
+}
+
+// src/scalars/microstructure.rs:117-135
+double msd(const double* c, unsigned R, double zeta, const std::vector<NlBlock>& b) {
+    const double e0 = predict_e0(c, R, zeta, b);
+    if (!std::isfinite(e0) || e0 == 0.0) return kNaN;
+    double acc = 0.0;
+    for (unsigned n = 0; n <= R; ++n) {
+        const double cn = c[n00_index(b, n)] / e0;
+        const double sign = (n % 2 == 0) ? 1.0 : -1.0;
+        const double w = std::sqrt(9.0 * gamma_fn(double(n) + 1.5) /
+                                   (8.0 * std::pow(kPi, 6) * std::pow(zeta, 3.5) * factorial(n)));
+        acc += cn * sign * w * hyp2f1(-double(n), 2.5, 1.5, 2.0);
+    }
+    return std::fmax(acc, 0.0);  // f64::max: a NaN operand yields the other
+}
+
+// src/scalars/microstructure.rs:150-177
+double qiv(const double* c, unsigned R, double zeta, const std::vector<NlBlock>& b) {
+    const double e0 = predict_e0(c, R, zeta, b);
+    if (!std::isfinite(e0) || e0 == 0.0) return kNaN;
+    const double pre = 6.0 * std::sqrt(2.0 * kPi) * std::pow(zeta, 2.5);
+    double integral = 0.0;
+    for (unsigned n = 0; n <= R; ++n) {
+        const double cn = c[n00_index(b, n)] / e0;
+        integral += cn * kappa(zeta, n, 0) * gamma_fn(double(n) + 1.5) / factorial(n) *
+                    hyp2f1(-double(n), 2.5, 1.5, 2.0);
+    }
+    const double q2 = pre * integral;
+    if (!std::isfinite(q2) || q2 <= 0.0) return kNaN;
+    return 1.0 / q2;
+}
+
+// src/scalars/microstructure.rs:181-197
+double ng(const double* c, int K, unsigned R, double zeta, const std::vector<NlBlock>& b) {
+    const double e0 = predict_e0(c, R, zeta, b);
+    if (!std::isfinite(e0) || e0 == 0.0) return kNaN;
+    double total = 0.0;
+    for (int i = 0; i < K; ++i) {
+        const double cn = c[i] / e0;
+        total += cn * cn;
+    }
+    if (total == 0.0) return 0.0;
+    const double c0 = c[0] / e0;
+    return std::sqrt(std::fmax(1.0 - (c0 * c0) / total, 0.0));
+}
+
+// src/scalars/microstructure.rs:278-284
+double alpha_factor(unsigned n, unsigned ell, double zeta) {
+    const double sign = ((ell / 2) % 2 == 0) ? 1.0 : -1.0;
+    const double num = 2.0 * kPi * std::pow(zeta, 1.5) * gamma_fn(double(n) + 1.5);
+    const double den = std::pow(2.0, int(ell)) * factorial(n - ell);
+    return sign * std::sqrt(num / den);
+}
+
+// src/scalars/microstructure.rs:291-306
+double rtap_kappa(unsigned j, unsigned ell) {
+    const double jf = double(j), lf = double(ell);
+    const double kap = std::pow(-1.0, int(j) - 1) * std::pow(2.0, -(lf + 3.0) / 2.0) / kPi;
+    double sum = 0.0;
+    for (unsigned k = 0; k < j; ++k) {
+        const double kf = double(k);
+        const double sign = (k % 2 == 0) ? 1.0 : -1.0;
+        sum += sign * binomial_float(jf + lf - 0.5, j - k - 1) * gamma_fn((lf + 1.0) / 2.0 + kf) /
+               (factorial(k) * std::pow(0.5, (lf + 1.0) / 2.0 + kf));
+    }
+    return 2.0 * kap * sum;
+}
+
+// src/scalars/microstructure.rs:309-322
+double rtpp_kappa(unsigned j, unsigned ell) {
+    const double jf = double(j), lf = double(ell);
+    const double pre = std::pow(-0.5, lf / 2.0) / std::sqrt(kPi);
+    double sum = 0.0;
+    for (unsigned k = 0; k < j; ++k) {
+        const double kf = double(k);
+        const double sign = (k % 2 == 0) ? 1.0 : -1.0;
+        sum += sign * binomial_float(jf + lf - 0.5, j - k - 1) * gamma_fn(lf / 2.0 + kf + 0.5) /
+               (factorial(k) * std::pow(0.5, lf / 2.0 + 0.5 + kf));
+    }
+    return pre * sum;
+}
+
+// src/scalars/microstructure.rs:219-270 — RTAP (axis = true) or RTPP.
+double directional(const double* c, unsigned R, double zeta, const std::vector<NlBlock>& b, const double* dir,
+                   bool axis) {
+    const double norm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (!std::isfinite(norm) || norm == 0.0) return kNaN;
+    const double e0 = predict_e0(c, R, zeta, b);
+    if (!std::isfinite(e0) || e0 == 0.0) return kNaN;
+    const double inv = 1.0 / norm;
+    double r, theta, phi;
+    cart2sphere(dir[0] * inv, dir[1] * inv, dir[2] * inv, &r, &theta, &phi);
+    double acc = 0.0;
+    std::vector<double> sh(2 * R + 1);
+    for (const auto& k : b) {
+        const unsigned j = k.n - k.ell + 1;
+        const double kern = axis ? rtap_kappa(j, k.ell) : rtpp_kappa(j, k.ell);
+        if (kern == 0.0) continue;
+        const double scale = kern / alpha_factor(k.n, k.ell, zeta);
+        brainsuite_sh_block(k.ell, theta, phi, sh.data());
+        double inner = 0.0;
+        for (int t = 0; t < k.len; ++t) inner += (c[k.start + t] / e0) * sh[t];
+        acc += scale * inner;
+    }
+    const double mu = 1.0 / (2.0 * kPi * std::sqrt(zeta));
+    const double pref = axis ? 1.0 / (mu * mu) : 1.0 / mu;
+    return std::fmax(pref * acc, 0.0);
+}
+
+// src/io/microstructure.rs:303-313 — index round((n-1) p) of the sorted finite values.
+bool percentile_value(const std::vector<float>& xs, double pct, float* out) {
+    std::vector<float> f;
+    for (float v : xs)
+        if (std::isfinite(v)) f.push_back(v);
+    if (f.empty()) return false;
+    std::sort(f.begin(), f.end());
+    const double p = std::min(std::max(pct, 0.0), 100.0) / 100.0;
+    const size_t i = std::min(size_t(std::llround(p * double(f.size() - 1))), f.size() - 1);
+    *out = f[i];
+    return true;
+}
+
+// src/io/microstructure.rs:317-326
+int nan_above(std::vector<float>& xs, float thr) {
+    int n = 0;
+    for (float& v : xs)
+        if (std::isfinite(v) && v > thr) {
+            v = std::numeric_limits<float>::quiet_NaN();
+            ++n;
+        }
+    return n;
+}
+
 }  // namespace
 
 // ======================================================== C entry points ====
@@ -799,6 +1053,101 @@ void orc_effective_table(const double* T, const double* R, const double* bvals, 
         b_out[i] = bvals[i] * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
         for (int k = 0; k < 3; ++k) dir_out[3 * i + k] = d[k] / dn;
     }
+}
+
+// ---------------------------------------------------- post-fit entry points ----
+double orc_hyp2f1(double a, double b, double c, double z) { return hyp2f1(a, b, c, z); }
+double orc_binomial_float(double n, unsigned k) { return binomial_float(n, k); }
+double orc_odf_radial_factor(unsigned n, unsigned ell, double zeta) { return odf_radial_factor(n, ell, zeta); }
+void orc_brainsuite_to_tournier(unsigned ell, int k, int* row, double* sign) { brainsuite_to_tournier(ell, k, row, sign); }
+void orc_tournier_matrix(unsigned radial_order, double zeta, unsigned lmax, double* out) {
+    tournier_matrix(radial_order, zeta, lmax, out);
+}
+double orc_alpha_factor(unsigned n, unsigned ell, double zeta) { return alpha_factor(n, ell, zeta); }
+
+// Raw per-voxel scalars (src/scalars/microstructure.rs:79-270) of one FP64
+// coefficient vector: out = rtop, rtap, rtpp, msd, qiv, ng. dir may be NULL
+// (RTAP/RTPP NaN, as for a missing peak).
+void orc_microstructure_raw(unsigned radial_order, double zeta, const double* c, const double* dir, double* out) {
+    const auto b = nl_blocks(radial_order);
+    const int K = int(shore_indices(radial_order).size());
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const double* d = dir ? dir : zero;
+    out[0] = rtop(c, radial_order, zeta, b);
+    out[1] = directional(c, radial_order, zeta, b, d, true);
+    out[2] = directional(c, radial_order, zeta, b, d, false);
+    out[3] = msd(c, radial_order, zeta, b);
+    out[4] = qiv(c, radial_order, zeta, b);
+    out[5] = ng(c, K, radial_order, zeta, b);
+}
+
+// src/io/microstructure.rs:160-300 (the values it returns): per masked voxel
+// (coefs V x K f32, dirs V x 3 f32 or NULL; a zero or non-finite direction is
+// a missing peak, python/src/models.rs:318-326), the six scalars as f32 times
+// the unit factor (units 0 = um, 1 = mm), then, when outlier_factor > 0, each
+// scalar's finite values above outlier_factor x p99 set to NaN.
+// out: 6 x V (rtop, rtap, rtpp, msd, qiv, ng).
+void orc_compute_microstructure(unsigned radial_order, double zeta, const float* coefs, const float* dirs, int64_t V,
+                                int units, float outlier_factor, float* out) {
+    const int K = int(shore_indices(radial_order).size());
+    const auto b = nl_blocks(radial_order);
+    const float su[6] = {1e-9f, 1e-6f, 1e-3f, 1e6f, 1e15f, 1.0f};
+    std::vector<std::vector<float>> vals(6, std::vector<float>(size_t(V)));
+    std::vector<double> c(K);
+    for (int64_t v = 0; v < V; ++v) {
+        for (int k = 0; k < K; ++k) c[k] = double(coefs[size_t(v) * K + k]);
+        double d[3] = {0.0, 0.0, 0.0};
+        if (dirs) {
+            const float* p = dirs + 3 * v;
+            const bool any = p[0] != 0.0f || p[1] != 0.0f || p[2] != 0.0f;
+            const bool fin = std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]);
+            if (any && fin)
+                for (int t = 0; t < 3; ++t) d[t] = double(p[t]);
+        }
+        double raw[6];
+        raw[0] = rtop(c.data(), radial_order, zeta, b);
+        raw[1] = directional(c.data(), radial_order, zeta, b, d, true);
+        raw[2] = directional(c.data(), radial_order, zeta, b, d, false);
+        raw[3] = msd(c.data(), radial_order, zeta, b);
+        raw[4] = qiv(c.data(), radial_order, zeta, b);
+        raw[5] = ng(c.data(), K, radial_order, zeta, b);
+        for (int q = 0; q < 6; ++q) vals[q][size_t(v)] = float(raw[q]) * (units == 0 ? su[q] : 1.0f);
+    }
+    if (outlier_factor > 0.0f)
+        for (int q = 0; q < 6; ++q) {
+            float p99;
+            if (percentile_value(vals[q], 99.0, &p99)) nan_above(vals[q], outlier_factor * p99);
+        }
+    for (int q = 0; q < 6; ++q) std::copy(vals[q].begin(), vals[q].end(), out + size_t(q) * V);
+}
+
+// src/synth.rs:32-84 — per voxel s = A c (FP64) as f32; an all-zero
+// coefficient vector gives zeros. A is the N' x K target design (row-major).
+void orc_synthesize(const double* A, int Np, int K, const float* coefs, int64_t V, float* out) {
+    for (int64_t v = 0; v < V; ++v) {
+        const float* c = coefs + size_t(v) * K;
+        bool any = false;
+        for (int k = 0; k < K; ++k) any = any || (c[k] != 0.0f);
+        for (int i = 0; i < Np; ++i) {
+            if (!any) {
+                out[size_t(v) * Np + i] = 0.0f;
+                continue;
+            }
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) acc += A[size_t(i) * K + k] * double(c[k]);
+            out[size_t(v) * Np + i] = float(acc);
+        }
+    }
+}
+
+// python/src/models.rs:228-247 — per voxel t = M c (FP64) as f32.
+void orc_project(const double* M, int rows, int K, const float* coefs, int64_t V, float* out) {
+    for (int64_t v = 0; v < V; ++v)
+        for (int i = 0; i < rows; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) acc += M[size_t(i) * K + k] * double(coefs[size_t(v) * K + k]);
+            out[size_t(v) * rows + i] = float(acc);
+        }
 }
 
 }  // extern "C"
