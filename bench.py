@@ -62,6 +62,9 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=32768)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-postfit", action="store_true", help="skip the post-fit map timings")
+    p.add_argument("--precision-report", action="store_true",
+                   help="only print the precision report of the 16x16x8 order-6 volume (device fits at FP32, "
+                        "BF16 and FP16 with refresh 5/10/20 against the FP64 oracle fit) and exit")
     p.add_argument("--max-iter", type=int, default=500)
     p.add_argument("--alpha-mode", default="fixed", choices=["fixed", "alpha-ratio", "path-bic", "l2-anchored"],
                    help="alpha strategy (the headline line is the fixed-alpha fit)")
@@ -81,6 +84,23 @@ def copy_calibration(mib: int):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration": {"bytes_read": n, "bytes_written": n, "launches": 3}}), flush=True)
+
+
+def precision_report():
+    from cs_dmri_amd import workloads as W
+    from cs_dmri_amd.precision import report
+    from oracle import load
+    wl = W.config1()
+    g = wl.gtab
+    orc = load()
+    A = orc.design_matrix(6, 700.0, g.bvals, g.bvecs, g.big_delta, g.small_delta)
+    t0 = time.perf_counter()
+    ref = orc.fit_volume(wl.data, wl.mask, A, alpha_mode="fixed", alpha=1e-3, max_iter=500, tol=1e-6,
+                         n_threads=min(os.cpu_count() or 1, 16))["coefficients"]
+    rep = report(wl.data, wl.mask, g, ref)
+    rep["workload"] = "shore6-cartesian101-16x16x8 (2048 voxels), fista-l1 alpha=1e-3, max_iter 500, tol 1e-6"
+    rep["fp64_reference_s"] = round(time.perf_counter() - t0, 2)
+    print(json.dumps({"precision_report": rep}), flush=True)
 
 
 def postfit_timings(L, d_coef, d_mask, wl, K, N, dev):
@@ -130,6 +150,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.precision_report:
+        precision_report()
         return
 
     import torch
