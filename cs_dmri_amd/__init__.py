@@ -4,8 +4,9 @@ The compute path is ``libshorefit.so`` (hand-written HIP kernels behind the C
 ABI in include/shorefit.h); there is no CPU fallback.
 """
 
-from .models import (design_matrix, fista_batch, n_coeffs, shore_fit, shore_microstructure,  # noqa: F401
-                     shore_odf_sh, shore_predict, tikhonov_batch, tournier_matrix)
+from .models import (design_matrix, fista_batch, graddev_gram, graddev_tune, graddev_variants,  # noqa: F401
+                     n_coeffs, shore_fit, shore_microstructure, shore_odf_sh, shore_predict, tikhonov_batch,
+                     tournier_matrix)
 from .shore import GradientTable, ShoreFit, ShoreModel  # noqa: F401
 
 __version__ = "0.1.0"

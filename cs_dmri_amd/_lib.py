@@ -25,6 +25,7 @@ EXPORTS = (
     "shorefit_spec_default", "shorefit_fit", "shorefit_n_coeffs", "shorefit_design_matrix",
     "shorefit_fista_batch", "shorefit_tikhonov_batch", "shorefit_last_error", "shorefit_version",
     "shorefit_tournier_matrix", "shorefit_odf_sh", "shorefit_predict", "shorefit_microstructure",
+    "shorefit_fit_graddev", "shorefit_graddev_variants", "shorefit_graddev_tune", "shorefit_graddev_gram",
 )
 
 
@@ -72,6 +73,13 @@ class ShorefitOutput(C.Structure):
         return [p.decode("utf-8") for p in parts[: self.n_warnings]]
 
 
+class ShorefitGradDev(C.Structure):
+    _fields_ = [
+        ("field", C.c_void_p), ("bvecs_vox", C.POINTER(C.c_double)), ("frame_rotation", C.c_double * 9),
+        ("variant", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 _pd = C.POINTER(C.c_double)
 _pf = C.POINTER(C.c_float)
 
@@ -114,6 +122,17 @@ def _load() -> C.CDLL:
     lib.shorefit_microstructure.argtypes = [_vp, _vp, C.c_int64, C.c_int64, C.c_int64, C.c_uint32, C.c_double, _vp,
                                             C.c_int32, C.c_float] + [_vp] * 6 + [C.c_int32, _vp]
     lib.shorefit_microstructure.restype = C.c_int
+    _pg, _pv, _pdv = C.POINTER(ShorefitGradients), C.POINTER(ShorefitVolume), C.POINTER(ShorefitGradDev)
+    lib.shorefit_fit_graddev.argtypes = [_pg, _pv, _pdv, C.POINTER(ShorefitSpec), C.POINTER(ShorefitOutput), _vp]
+    lib.shorefit_fit_graddev.restype = C.c_int
+    _pi = C.POINTER(C.c_int32)
+    lib.shorefit_graddev_variants.argtypes = [C.c_uint32, C.c_int32, _pi, _pi, _pi, _pi]
+    lib.shorefit_graddev_variants.restype = C.c_int
+    lib.shorefit_graddev_tune.argtypes = [_pg, _pv, _pdv, C.POINTER(ShorefitSpec), C.c_int64, _pd, _vp]
+    lib.shorefit_graddev_tune.restype = C.c_int
+    lib.shorefit_graddev_gram.argtypes = [_pg, _pv, _pdv, C.c_uint32, C.c_double, C.POINTER(C.c_int64), C.c_int64,
+                                          _pf, _pd, _pd]
+    lib.shorefit_graddev_gram.restype = C.c_int
     return lib
 
 

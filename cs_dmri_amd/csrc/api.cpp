@@ -6,7 +6,9 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <cstdlib>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -458,6 +460,530 @@ int64_t voxel_list(const uint8_t* d_mask, int64_t n, DevBuf<int64_t>& vox, hipSt
 void check_dims(int64_t nx, int64_t ny, int64_t nz) {
     if (nx < 0 || ny < 0 || nz < 0) throw Error{SHOREFIT_E_INVALID, "negative volume dimension"};
 }
+// ------------------------------------------------ gradient-deviation fits --
+// Tuned variant per (radial order, N), filled by shorefit_graddev_tune.
+std::mutex& tuned_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::map<std::pair<uint32_t, int64_t>, int>& tuned_variants() {
+    static std::map<std::pair<uint32_t, int64_t>, int> t;
+    return t;
+}
+
+// The requested variant, else the tuned one for (order, N), else the first
+// (the default) of the order.
+const GdVariant& pick_variant(uint32_t order, int64_t N, int32_t requested) {
+    const GdVariant* vs = nullptr;
+    const int nv = gd_variants(order, &vs);
+    if (nv == 0)
+        throw Error{SHOREFIT_E_UNSUPPORTED,
+                    "gradient-deviation fits support even radial orders 2 to 10, got " + std::to_string(order)};
+    if (requested >= 0) {
+        if (requested >= nv)
+            throw Error{SHOREFIT_E_INVALID, "variant " + std::to_string(requested) + " out of range (radial order " +
+                                                std::to_string(order) + " has " + std::to_string(nv) + ")"};
+        return vs[requested];
+    }
+    std::lock_guard<std::mutex> lk(tuned_mutex());
+    const auto it = tuned_variants().find({order, N});
+    return it != tuned_variants().end() ? vs[it->second] : vs[0];
+}
+
+// Identity voxels (max |T - I| < 1e-6) of the voxel list into id, the others
+// into dv; both in list order. Returns the identity count.
+int64_t gd_partition(const float* field, const int64_t* vox, int64_t V, DevBuf<int64_t>& id, DevBuf<int64_t>& dv,
+                     int64_t* n_dv, hipStream_t s) {
+    DevBuf<uint8_t> fid(V), fdv(V);
+    gd_launch_identity(field, vox, V, fid.p, fdv.p, s);
+    DevBuf<int64_t> pid, pdv;
+    const int64_t n_id = V ? compact_mask(fid.p, V, pid, s) : 0;
+    *n_dv = V ? compact_mask(fdv.p, V, pdv, s) : 0;
+    id.alloc(std::max<int64_t>(n_id, 1));
+    dv.alloc(std::max<int64_t>(*n_dv, 1));
+    launch_gather_i64(vox, pid.p, n_id, id.p, s);
+    launch_gather_i64(vox, pdv.p, *n_dv, dv.p, s);
+    return n_id;
+}
+
+struct GdSetup {
+    GdBasis B;
+    GdTable tab;
+    DevBuf<double> bvals, gvox, gnom;
+};
+
+void gd_setup(GdSetup& S, const ShorefitGradients& g, double big, double small, const ShorefitGradDev& gd,
+              uint32_t order, double zeta, hipStream_t s) {
+    const int N = int(g.n_grads);
+    S.B = gd_basis_host(order, zeta);
+    S.bvals.alloc(N);
+    S.gvox.alloc(size_t(3) * N);
+    S.gnom.alloc(size_t(3) * N);
+    SF_HIP(hipMemcpyAsync(S.bvals.p, g.bvals, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(S.gvox.p, gd.bvecs_vox ? gd.bvecs_vox : g.bvecs, sizeof(double) * 3 * N,
+                          hipMemcpyHostToDevice, s));
+    SF_HIP(hipMemcpyAsync(S.gnom.p, g.bvecs, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+    S.tab.bvals = S.bvals.p;
+    S.tab.gvox = S.gvox.p;
+    S.tab.gnom = S.gnom.p;
+    for (int k = 0; k < 9; ++k) S.tab.R[k] = gd.frame_rotation[k];
+    S.tab.b0_thr = g.b0_threshold;
+    const double tau = big - small / 3.0;
+    S.tab.qdenom = 4.0 * kPi * kPi * tau;  // src/qspace.rs:130-149
+    S.tab.N = N;
+}
+
+struct GdRunOut {
+    float* coef;
+    float* resid;
+    float* r2;
+    uint32_t* iters;
+    uint8_t* kind;
+    float* alpha_map;
+    float* bic;
+};
+
+// Workspace per chunk: G_v (FP32 Kp x Kp) and b_v (FP64) of every voxel of
+// the chunk; 8 GiB unless SHOREFIT_GRADDEV_WORKSPACE_MB says otherwise.
+int64_t gd_chunk_voxels(const GdVariant& v, int64_t V) {
+    size_t cap = size_t(8) << 30;
+    if (const char* e = std::getenv("SHOREFIT_GRADDEV_WORKSPACE_MB")) {
+        const long long mb = std::atoll(e);
+        if (mb > 0) cap = size_t(mb) << 20;
+    }
+    const size_t per = size_t(v.kp) * v.kp * sizeof(float) + size_t(v.kp) * sizeof(double) + 16;
+    return std::max<int64_t>(1, std::min<int64_t>(V, int64_t(cap / per)));
+}
+
+// The deviated voxels vox[0..V): Gram, FISTA and residual kernels per chunk.
+// lip (optional, device, V entries) receives each voxel's Lipschitz constant.
+void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, const float* data, int64_t sig_stride,
+            const float* field, const int64_t* vox, int64_t V, const GdRunOut& o, double* lip, hipStream_t s) {
+    if (V <= 0) return;
+    const int N = S.tab.N, K = S.B.K, KP = var.kp;
+    // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112)
+    std::vector<float> mom(std::max<uint32_t>(sp.max_iter, 1));
+    double t = 1.0;
+    for (size_t k = 0; k < mom.size(); ++k) {
+        const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+        mom[k] = float((t - 1.0) / tn);
+        t = tn;
+    }
+    DevBuf<float> d_mom(mom.size());
+    SF_HIP(hipMemcpyAsync(d_mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
+    const int64_t Vb = gd_chunk_voxels(var, V);
+    DevBuf<float> G(size_t(Vb) * KP * KP), av(Vb);
+    DevBuf<double> b(size_t(Vb) * KP);
+    DevBuf<uint32_t> itsb(Vb);
+    DevBuf<uint8_t> st(Vb);
+    DevBuf<unsigned> ctr(1);
+    for (int64_t c0 = 0; c0 < V; c0 += Vb) {
+        const int64_t n = std::min(Vb, V - c0);
+        GdChunk c{S.B, S.tab, data, sig_stride, field, vox + c0, n, G.p, b.p};
+        gd_launch_gram(var, c, s);
+        SF_HIP(hipMemsetAsync(ctr.p, 0, sizeof(unsigned), s));
+        GdFistaArgs fa{};
+        fa.G = G.p;
+        fa.b = b.p;
+        fa.vox = vox + c0;
+        fa.n = n;
+        fa.N = N;
+        fa.K = K;
+        fa.alpha = sp.alpha;
+        fa.ratio = sp.alpha_mode == SHOREFIT_ALPHA_RATIO ? sp.alpha_ratio : 0.0;
+        fa.max_iter = sp.max_iter;
+        fa.tol = sp.tol;
+        fa.nonneg = sp.non_negative;
+        fa.mom = d_mom.p;
+        fa.counter = ctr.p;
+        fa.coef = o.coef;
+        fa.its = itsb.p;
+        fa.status = st.p;
+        fa.alpha_v = av.p;
+        fa.lip = lip ? lip + c0 : nullptr;
+        gd_launch_fista(var, fa, 0, s);
+        if (!o.resid && !o.r2 && !o.iters && !o.kind && !o.alpha_map && !o.bic) continue;
+        GdOutArgs oa{o.coef, itsb.p, st.p, av.p, o.resid, o.r2, o.iters, nullptr, o.kind, o.alpha_map, o.bic};
+        gd_launch_residual(c, oa, s);
+    }
+    SF_HIP(hipStreamSynchronize(s));  // the host momentum table goes out of scope
+}
+
+// The volume fit behind shorefit_fit and shorefit_fit_graddev: validation,
+// gradient table, voxel list, the shared-design solve and, with a
+// gradient-deviation field, the per-voxel-design solve of the deviated
+// voxels.
+void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitSpec* spec,
+              ShorefitOutput* out, hipStream_t s, const ShorefitGradDev* gd) {
+    const ShorefitSpec& sp = *spec;
+    Warnings warn;
+
+    // --- spec validation (build_alpha_strategy, src/fit.rs:555-591; the
+    // binding's own checks, python/src/models.rs:117-147) -------------
+    if (sp.regularization == SHOREFIT_REG_NONNEG)
+        throw Error{SHOREFIT_E_UNSUPPORTED, "regularization 'nonneg' (ICLS) is not implemented on this device"};
+    if (sp.regularization != SHOREFIT_REG_L1 && sp.regularization != SHOREFIT_REG_L2)
+        throw Error{SHOREFIT_E_INVALID, "regularization must be l1, l2 or nonneg"};
+    double path_eps = sp.path_eps;
+    if (sp.regularization == SHOREFIT_REG_L1) {
+        switch (sp.alpha_mode) {
+            case SHOREFIT_ALPHA_FIXED: break;
+            case SHOREFIT_ALPHA_RATIO:
+                if (!(sp.alpha_ratio > 0.0 && sp.alpha_ratio < 1.0))
+                    throw Error{SHOREFIT_E_INVALID,
+                                "--alpha-ratio must lie in (0, 1), got " + std::to_string(sp.alpha_ratio)};
+                break;
+            case SHOREFIT_ALPHA_PATH_BIC:
+            case SHOREFIT_ALPHA_L2_ANCHORED: {
+                if (sp.path_n_alphas < 2)
+                    throw Error{SHOREFIT_E_INVALID,
+                                "--path-n-alphas must be ≥ 2, got " + std::to_string(sp.path_n_alphas)};
+                if (!(path_eps > 0.0)) path_eps = sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC ? 1e-3 : 1e-4;
+                if (!(path_eps > 0.0 && path_eps < 1.0))
+                    throw Error{SHOREFIT_E_INVALID, "--path-eps must lie in (0, 1), got " + std::to_string(path_eps)};
+                if (sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED && sp.slack < 0.0)
+                    throw Error{SHOREFIT_E_INVALID, "--slack must be ≥ 0, got " + std::to_string(sp.slack)};
+                break;
+            }
+            default:
+                throw Error{SHOREFIT_E_INVALID, "alpha_mode must be fixed, alpha-ratio, path-bic or l2-anchored"};
+        }
+    }
+    if (sp.radial_order % 2 != 0 || sp.radial_order == 0)
+        throw Error{SHOREFIT_E_INVALID,
+                    "radial_order must be a positive even number, got " + std::to_string(sp.radial_order)};
+    if (sp.precision != SHOREFIT_PREC_FP32 && sp.precision != SHOREFIT_PREC_FP16 &&
+        sp.precision != SHOREFIT_PREC_BF16)
+        throw Error{SHOREFIT_E_INVALID, "precision must be fp32, fp16 or bf16"};
+    const GdVariant* gd_var = nullptr;
+    if (gd) {
+        if (!gd->field) throw Error{SHOREFIT_E_INVALID, "the gradient-deviation field is required"};
+        if (sp.regularization != SHOREFIT_REG_L1 ||
+            (sp.alpha_mode != SHOREFIT_ALPHA_FIXED && sp.alpha_mode != SHOREFIT_ALPHA_RATIO))
+            throw Error{SHOREFIT_E_UNSUPPORTED,
+                        "gradient-deviation fits support regularization l1 with alpha_mode fixed or alpha-ratio"};
+        gd_var = &pick_variant(sp.radial_order, grads->n_grads, gd->variant);
+    }
+
+    // --- gradient table (src/qspace.rs:90-120) ------------------------
+    const int64_t N = grads->n_grads;
+    if (N <= 0 || !grads->bvals || !grads->bvecs) throw Error{SHOREFIT_E_INVALID, "empty gradient table"};
+    if (vol->nx < 0 || vol->ny < 0 || vol->nz < 0) throw Error{SHOREFIT_E_INVALID, "negative volume shape"};
+    double big = grads->big_delta, small = grads->small_delta;
+    out->delta_source = 0;
+    if (std::isnan(big) || std::isnan(small)) {
+        tortoise_deltas(grads->bvals, N, &big, &small);
+        out->delta_source = 1;
+    }
+    out->big_delta = big;
+    out->small_delta = small;
+    out->tau = big - small / 3.0;
+
+    // --- fit_shore refusals and warnings (src/fit.rs:691-730) ---------
+    const auto shells = dwi_shells(grads->bvals, N, grads->b0_threshold);
+    out->n_shells = int32_t(shells.size());
+    for (size_t i = 0; i < shells.size() && i < 32; ++i) out->dwi_shells[i] = shells[i];
+    if (shells.empty())
+        throw Error{SHOREFIT_E_FIT,
+                    "3D-SHORE needs diffusion-weighted data; every volume is at or below the b=0 threshold"};
+    if (shells.size() < kMinShoreShells) {
+        const std::string what = shells.size() == 1 ? "a single shell (b = " + fmt_b(shells[0]) + " s/mm²)"
+                                                    : std::to_string(shells.size()) + " shells";
+        if (!sp.allow_single_shell)
+            throw Error{
+                SHOREFIT_E_FIT,
+                "3D-SHORE needs diffusion-weighted data on at least 2 b-value shells; this series has " + what +
+                    ". With a single shell the radial decay of the signal is not determined by the data: a fit "
+                    "would match the measurements, but its propagator-derived scalars (RTOP, RTAP, RTPP, MSD, "
+                    "QIV) and the signals it predicts at other b-values would reflect the regularization rather "
+                    "than the data. For single-shell data use single-shell three-tissue CSD (cs-ss3t-full, "
+                    "cs_dmri.ss3t_pipeline). To fit anyway for orientation information only, allow single-shell "
+                    "data (--allow-single-shell, or allow_single_shell=True in Python)."};
+        warn.msgs.push_back("the data have " + what +
+                            ": only the orientation information of this fit is meaningful, and "
+                            "propagator-derived scalars are not computed");
+    }
+    const int K = n_coeffs(sp.radial_order);
+    out->n_coeffs = K;
+    if (N < K && sp.regularization != SHOREFIT_REG_L1)
+        warn.msgs.push_back(std::to_string(N) + " measurements for " + std::to_string(K) +
+                            " coefficients (radial order " + std::to_string(sp.radial_order) +
+                            "): without L1 regularization the fit is underdetermined and its fit statistics "
+                            "are not meaningful; use a lower radial order or L1");
+
+    // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
+    // A device mask is compacted on the device; a host mask on the host;
+    // without a mask every voxel is fitted in place (no index list).
+    const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
+    DevBuf<int64_t> d_vox;
+    int64_t V = 0;
+    if (!vol->mask) {
+        V = nvox_total;
+    } else if (vol->on_device) {
+        V = compact_mask(vol->mask, nvox_total, d_vox, s);
+    } else {
+        std::vector<int64_t> vox;
+        vox.reserve(nvox_total);
+        for (int64_t i = 0; i < nvox_total; ++i)
+            if (vol->mask[i]) vox.push_back(i);
+        V = int64_t(vox.size());
+        d_vox.alloc(V);
+        if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
+        SF_HIP(hipStreamSynchronize(s));  // vox is released at scope exit
+    }
+    const int64_t* vox_dev = d_vox.p;  // null: voxel v is row v
+    out->n_voxels = V;
+
+    // --- gradient deviation: identity voxels keep the shared design -------
+    const int64_t* vox_shared = vox_dev;
+    int64_t V_shared = V;
+    DevBuf<float> d_field;
+    DevBuf<int64_t> d_vox_id, d_vox_dv;
+    const float* field = nullptr;
+    int64_t V_dv = 0;
+    if (gd) {
+        field = dev_in(gd->field, size_t(nvox_total) * 9, vol->on_device != 0, d_field, s);
+        V_shared = gd_partition(field, vox_dev, V, d_vox_id, d_vox_dv, &V_dv, s);
+        vox_shared = d_vox_id.p;
+    }
+
+    // --- device buffers -------------------------------------------------
+    const size_t coef_bytes = sizeof(float) * size_t(nvox_total) * K;
+    DevBuf<float> d_data, d_coef, d_r2, d_res, d_alpha, d_bic, d_rssl2;
+    DevBuf<uint32_t> d_it;
+    DevBuf<uint8_t> d_kind;
+    const bool dev = vol->on_device != 0;
+    const float* data = vol->data;
+    if (!dev) {
+        d_data.alloc(size_t(nvox_total) * N);
+        if (nvox_total)
+            SF_HIP(hipMemcpyAsync(d_data.p, vol->data, sizeof(float) * nvox_total * N, hipMemcpyHostToDevice, s));
+        data = d_data.p;
+    }
+    auto out_buf = [&](auto* user, auto& buf, size_t count) {
+        using T = std::remove_pointer_t<decltype(user)>;
+        if (!user) return static_cast<T*>(nullptr);
+        if (dev) return user;
+        buf.alloc(count);
+        return buf.p;
+    };
+    const bool l1 = sp.regularization == SHOREFIT_REG_L1;
+    float* coef = out_buf(out->coefficients, d_coef, size_t(nvox_total) * K);
+    float* r2 = out_buf(out->r2, d_r2, nvox_total);
+    float* res = out_buf(out->residual_l2, d_res, nvox_total);
+    uint32_t* its = out_buf(out->iterations, d_it, nvox_total);
+    uint8_t* kind = out_buf(out->regularization_kind, d_kind, nvox_total);
+    float* amap = l1 ? out_buf(out->alpha, d_alpha, nvox_total) : nullptr;
+    float* bic = l1 ? out_buf(out->bic, d_bic, nvox_total) : nullptr;
+    const bool path = l1 && (sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC || sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED);
+    const bool anchored = l1 && sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED;
+    // L2-anchored: the Tikhonov reference RSS per voxel (also the rss_l2 map)
+    float* rssl2 = anchored ? out_buf(out->rss_l2, d_rssl2, nvox_total) : nullptr;
+    if (anchored && !rssl2) {
+        d_rssl2.alloc(nvox_total);
+        rssl2 = d_rssl2.p;
+    }
+    DevBuf<float> d_alpha_tmp;
+    const bool per_voxel_alpha = l1 && sp.alpha_mode != SHOREFIT_ALPHA_FIXED;
+    if (per_voxel_alpha && !amap) {
+        d_alpha_tmp.alloc(nvox_total);
+        amap = d_alpha_tmp.p;
+    }
+    // zeros outside the mask (src/fit.rs:89-101)
+    if (nvox_total) {
+        SF_HIP(hipMemsetAsync(coef, 0, coef_bytes, s));
+        if (r2) SF_HIP(hipMemsetAsync(r2, 0, sizeof(float) * nvox_total, s));
+        if (res) SF_HIP(hipMemsetAsync(res, 0, sizeof(float) * nvox_total, s));
+        if (its) SF_HIP(hipMemsetAsync(its, 0, sizeof(uint32_t) * nvox_total, s));
+        if (kind) SF_HIP(hipMemsetAsync(kind, 0, nvox_total, s));
+        if (amap) SF_HIP(hipMemsetAsync(amap, 0, sizeof(float) * nvox_total, s));
+        if (bic) SF_HIP(hipMemsetAsync(bic, 0, sizeof(float) * nvox_total, s));
+    }
+    if (out->rss_l2 && dev && nvox_total) SF_HIP(hipMemsetAsync(out->rss_l2, 0, sizeof(float) * nvox_total, s));
+    if (rssl2 && nvox_total) SF_HIP(hipMemsetAsync(rssl2, 0, sizeof(float) * nvox_total, s));
+
+    // --- design + solver set-up (src/fit.rs:718-759) -------------------
+    const int precision = (l1 && sp.max_iter > 0) ? sp.precision : SHOREFIT_PREC_FP32;
+    Design d;
+    build_design(d, *grads, big, small, sp.radial_order, sp.zeta, s, precision != SHOREFIT_PREC_FP32);
+    hipEvent_t e0, e1, e2, e3;
+    SF_HIP(hipEventCreate(&e0));
+    SF_HIP(hipEventCreate(&e1));
+    SF_HIP(hipEventCreate(&e2));
+    SF_HIP(hipEventCreate(&e3));
+    if (l1) {
+        FistaOps ops;
+        FistaLaunch L{};
+        FistaWork work;
+        L.data = data;
+        L.vox = vox_shared;
+        L.n_vox = V_shared;
+        L.N = d.N;
+        L.sig_stride = N;
+        L.K = d.K;
+        L.Kp = d.Kp;
+        L.Np = d.Np;
+        L.ratio = sp.alpha_mode == SHOREFIT_ALPHA_RATIO ? float(sp.alpha_ratio) : 0.f;
+        L.alpha = sp.alpha;
+        L.max_iter = sp.max_iter;
+        L.tol = float(sp.tol);
+        L.nonneg = sp.non_negative;
+        L.precision = precision;
+        L.refresh = std::max(1, sp.refresh_every);
+        L.coef = coef;
+        L.resid = res;
+        L.r2 = r2;
+        L.iters = its;
+        L.kind = kind;
+        L.alpha_map = amap;
+        L.bic = bic;
+        // alpha path: f_i = exp(t_i ln eps), t_i = i / (n - 1), so that
+        // alpha_i = alpha_max f_i (src/solver/alpha.rs:57-69)
+        DevBuf<float> d_pf;
+        DevBuf<double> d_pf64;
+        TikOps tik;
+        if (path) {
+            double eps = sp.path_eps;
+            if (!(eps > 0.0)) eps = sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC ? 1e-3 : 1e-4;
+            const int n = int(sp.path_n_alphas);
+            std::vector<double> f64(n);
+            std::vector<float> f32(n);
+            for (int i = 0; i < n; ++i) {
+                f64[i] = std::exp(double(i) / double(n - 1) * std::log(eps));
+                f32[i] = float(f64[i]);
+            }
+            d_pf.alloc(n);
+            d_pf64.alloc(n);
+            SF_HIP(hipMemcpyAsync(d_pf.p, f32.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+            SF_HIP(hipMemcpyAsync(d_pf64.p, f64.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+            SF_HIP(hipStreamSynchronize(s));
+            L.path_n = n;
+            L.path_f = d_pf.p;
+            L.path_f64 = d_pf64.p;
+            L.path_rule = anchored ? 1 : 0;
+            L.slack = sp.slack;
+            L.rss_l2_ref = rssl2;
+        }
+        setup_fista(d, ops, work, L, s);
+        out->lipschitz = d.L;
+        L.thr = float(ops.step * sp.alpha);
+        if (anchored && V > 0) {
+            // the Tikhonov reference fit (src/fit.rs:404-410, 760-761)
+            std::vector<double> dn, dl;
+            regularization_diagonals(sp.radial_order, dn, dl);
+            pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, tik, s);
+            TikhonovLaunch T{};
+            T.data = data;
+            T.vox = vox_dev;
+            T.n_vox = V;
+            T.N = d.N;
+            T.sig_stride = N;
+            T.K = d.K;
+            T.Kp = d.Kp;
+            T.Np = d.Np;
+            T.Pfrag = tik.P.p;
+            T.Afrag = tik.A.p;
+            T.rss_l2 = rssl2;
+            launch_tikhonov(T, s);
+        }
+        SF_HIP(hipEventRecord(e0, s));
+        L.event_mid = e1;  // recorded right after the solve kernel
+        launch_fista(L, s);
+        if (V_shared == 0) SF_HIP(hipEventRecord(e1, s));
+        SF_HIP(hipEventRecord(e2, s));
+        if (gd && V_dv > 0) {
+            // deviated voxels: per-voxel designs (src/graddev.rs:160-185)
+            GdSetup S;
+            gd_setup(S, *grads, big, small, *gd, sp.radial_order, sp.zeta, s);
+            GdRunOut o{coef, res, r2, its, kind, amap, bic};
+            gd_run(S, *gd_var, sp, data, N, field, d_vox_dv.p, V_dv, o, nullptr, s);
+        }
+        SF_HIP(hipEventRecord(e3, s));
+    } else {
+        d.G.alloc(size_t(d.K) * d.K);
+        launch_gram(d.A.p, d.N, d.K, d.G.p, s);
+        std::vector<double> dn, dl;
+        regularization_diagonals(sp.radial_order, dn, dl);
+        TikOps ops;
+        pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, ops, s);
+        TikhonovLaunch L{};
+        L.data = data;
+        L.vox = vox_dev;
+        L.n_vox = V;
+        L.N = d.N;
+        L.sig_stride = N;
+        L.K = d.K;
+        L.Kp = d.Kp;
+        L.Np = d.Np;
+        L.Pfrag = ops.P.p;
+        L.Afrag = ops.A.p;
+        L.coef = coef;
+        L.resid = res;
+        L.r2 = r2;
+        L.iters = its;
+        L.kind = kind;
+        SF_HIP(hipEventRecord(e0, s));
+        launch_tikhonov(L, s);
+        SF_HIP(hipEventRecord(e1, s));
+        SF_HIP(hipEventRecord(e2, s));
+        SF_HIP(hipEventRecord(e3, s));
+    }
+    SF_HIP(hipStreamSynchronize(s));
+    float ms = 0.f, ms_gd = 0.f;
+    SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    SF_HIP(hipEventElapsedTime(&ms_gd, e2, e3));
+    out->fit_ms = double(ms) + double(ms_gd);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(e3);
+
+    // --- alpha distribution (src/fit.rs:334-352) ------------------------
+    out->has_alpha_distribution = 0;
+    if (per_voxel_alpha) {
+        // the masked voxels' alphas in work order, then three order
+        // statistics (the values a full sort would put at those ranks)
+        std::vector<float> a_v(V);
+        if (V) {
+            DevBuf<float> d_av(V);
+            launch_gather_f32(amap, vox_dev, V, d_av.p, s);
+            SF_HIP(hipMemcpyAsync(a_v.data(), d_av.p, sizeof(float) * V, hipMemcpyDeviceToHost, s));
+            SF_HIP(hipStreamSynchronize(s));
+        }
+        std::vector<double> seen(a_v.begin(), a_v.end());
+        out->has_alpha_distribution = 1;
+        if (seen.empty()) {
+            out->alpha_distribution[0] = out->alpha_distribution[1] = out->alpha_distribution[2] = 0.0;
+        } else {
+            auto pct = [&](double q) {
+                size_t i = std::min(size_t(std::llround(double(seen.size() - 1) * q)), seen.size() - 1);
+                std::nth_element(seen.begin(), seen.begin() + i, seen.end());
+                return seen[i];
+            };
+            out->alpha_distribution[0] = pct(0.5);
+            out->alpha_distribution[1] = pct(0.10);
+            out->alpha_distribution[2] = pct(0.90);
+        }
+    }
+
+    // --- results back to the caller -------------------------------------
+    if (!dev && nvox_total) {
+        SF_HIP(hipMemcpy(out->coefficients, coef, coef_bytes, hipMemcpyDeviceToHost));
+        if (out->r2) SF_HIP(hipMemcpy(out->r2, r2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+        if (out->residual_l2)
+            SF_HIP(hipMemcpy(out->residual_l2, res, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+        if (out->iterations)
+            SF_HIP(hipMemcpy(out->iterations, its, sizeof(uint32_t) * nvox_total, hipMemcpyDeviceToHost));
+        if (out->regularization_kind)
+            SF_HIP(hipMemcpy(out->regularization_kind, kind, nvox_total, hipMemcpyDeviceToHost));
+        if (l1 && out->alpha) SF_HIP(hipMemcpy(out->alpha, amap, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+        if (l1 && out->bic) SF_HIP(hipMemcpy(out->bic, bic, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+        if (anchored && out->rss_l2)
+            SF_HIP(hipMemcpy(out->rss_l2, rssl2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+    }
+    warn.to(out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -606,335 +1132,171 @@ int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, cons
         if (!out->coefficients) throw Error{SHOREFIT_E_INVALID, "coefficients output is required"};
         hipStream_t s = as_stream(stream);
         StreamScope scope(s);
+        fit_impl(grads, vol, spec, out, s, nullptr);
+    });
+}
+
+int shorefit_fit_graddev(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
+                         const ShorefitSpec* spec, ShorefitOutput* out, void* stream) {
+    return guarded([&] {
+        if (!grads || !vol || !spec || !out || !gd) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        if (!out->coefficients) throw Error{SHOREFIT_E_INVALID, "coefficients output is required"};
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        fit_impl(grads, vol, spec, out, s, gd);
+    });
+}
+
+int shorefit_graddev_variants(uint32_t radial_order, int32_t i, int32_t* kp, int32_t* rows, int32_t* gram_waves,
+                              int32_t* fista_waves) {
+    const GdVariant* vs = nullptr;
+    const int nv = gd_variants(radial_order, &vs);
+    if (i >= 0 && i < nv) {
+        if (kp) *kp = vs[i].kp;
+        if (rows) *rows = vs[i].rc;
+        if (gram_waves) *gram_waves = vs[i].gram_waves;
+        if (fista_waves) *fista_waves = vs[i].fista_waves;
+    }
+    return nv;
+}
+
+int shorefit_graddev_tune(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
+                          const ShorefitSpec* spec, int64_t max_voxels, double* ms_out, void* stream) {
+    int best = -1;
+    const int rc = guarded([&] {
+        if (!grads || !vol || !spec || !gd || !gd->field) throw Error{SHOREFIT_E_INVALID, "null argument"};
         const ShorefitSpec& sp = *spec;
-        Warnings warn;
-
-        // --- spec validation (build_alpha_strategy, src/fit.rs:555-591; the
-        // binding's own checks, python/src/models.rs:117-147) -------------
-        if (sp.regularization == SHOREFIT_REG_NONNEG)
-            throw Error{SHOREFIT_E_UNSUPPORTED, "regularization 'nonneg' (ICLS) is not implemented on this device"};
-        if (sp.regularization != SHOREFIT_REG_L1 && sp.regularization != SHOREFIT_REG_L2)
-            throw Error{SHOREFIT_E_INVALID, "regularization must be l1, l2 or nonneg"};
-        double path_eps = sp.path_eps;
-        if (sp.regularization == SHOREFIT_REG_L1) {
-            switch (sp.alpha_mode) {
-                case SHOREFIT_ALPHA_FIXED: break;
-                case SHOREFIT_ALPHA_RATIO:
-                    if (!(sp.alpha_ratio > 0.0 && sp.alpha_ratio < 1.0))
-                        throw Error{SHOREFIT_E_INVALID,
-                                    "--alpha-ratio must lie in (0, 1), got " + std::to_string(sp.alpha_ratio)};
-                    break;
-                case SHOREFIT_ALPHA_PATH_BIC:
-                case SHOREFIT_ALPHA_L2_ANCHORED: {
-                    if (sp.path_n_alphas < 2)
-                        throw Error{SHOREFIT_E_INVALID,
-                                    "--path-n-alphas must be ≥ 2, got " + std::to_string(sp.path_n_alphas)};
-                    if (!(path_eps > 0.0)) path_eps = sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC ? 1e-3 : 1e-4;
-                    if (!(path_eps > 0.0 && path_eps < 1.0))
-                        throw Error{SHOREFIT_E_INVALID, "--path-eps must lie in (0, 1), got " + std::to_string(path_eps)};
-                    if (sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED && sp.slack < 0.0)
-                        throw Error{SHOREFIT_E_INVALID, "--slack must be ≥ 0, got " + std::to_string(sp.slack)};
-                    break;
-                }
-                default:
-                    throw Error{SHOREFIT_E_INVALID, "alpha_mode must be fixed, alpha-ratio, path-bic or l2-anchored"};
-            }
-        }
-        if (sp.radial_order % 2 != 0 || sp.radial_order == 0)
-            throw Error{SHOREFIT_E_INVALID,
-                        "radial_order must be a positive even number, got " + std::to_string(sp.radial_order)};
-        if (sp.precision != SHOREFIT_PREC_FP32 && sp.precision != SHOREFIT_PREC_FP16 &&
-            sp.precision != SHOREFIT_PREC_BF16)
-            throw Error{SHOREFIT_E_INVALID, "precision must be fp32, fp16 or bf16"};
-
-        // --- gradient table (src/qspace.rs:90-120) ------------------------
+        if (sp.regularization != SHOREFIT_REG_L1 ||
+            (sp.alpha_mode != SHOREFIT_ALPHA_FIXED && sp.alpha_mode != SHOREFIT_ALPHA_RATIO))
+            throw Error{SHOREFIT_E_UNSUPPORTED,
+                        "gradient-deviation fits support regularization l1 with alpha_mode fixed or alpha-ratio"};
+        const GdVariant* vs = nullptr;
+        const int nv = gd_variants(sp.radial_order, &vs);
+        if (nv == 0) throw Error{SHOREFIT_E_UNSUPPORTED, "gradient-deviation fits support even radial orders 2 to 10"};
+        check_dims(vol->nx, vol->ny, vol->nz);
         const int64_t N = grads->n_grads;
         if (N <= 0 || !grads->bvals || !grads->bvecs) throw Error{SHOREFIT_E_INVALID, "empty gradient table"};
-        if (vol->nx < 0 || vol->ny < 0 || vol->nz < 0) throw Error{SHOREFIT_E_INVALID, "negative volume shape"};
         double big = grads->big_delta, small = grads->small_delta;
-        out->delta_source = 0;
-        if (std::isnan(big) || std::isnan(small)) {
-            tortoise_deltas(grads->bvals, N, &big, &small);
-            out->delta_source = 1;
-        }
-        out->big_delta = big;
-        out->small_delta = small;
-        out->tau = big - small / 3.0;
-
-        // --- fit_shore refusals and warnings (src/fit.rs:691-730) ---------
-        const auto shells = dwi_shells(grads->bvals, N, grads->b0_threshold);
-        out->n_shells = int32_t(shells.size());
-        for (size_t i = 0; i < shells.size() && i < 32; ++i) out->dwi_shells[i] = shells[i];
-        if (shells.empty())
-            throw Error{SHOREFIT_E_FIT,
-                        "3D-SHORE needs diffusion-weighted data; every volume is at or below the b=0 threshold"};
-        if (shells.size() < kMinShoreShells) {
-            const std::string what = shells.size() == 1 ? "a single shell (b = " + fmt_b(shells[0]) + " s/mm²)"
-                                                        : std::to_string(shells.size()) + " shells";
-            if (!sp.allow_single_shell)
-                throw Error{
-                    SHOREFIT_E_FIT,
-                    "3D-SHORE needs diffusion-weighted data on at least 2 b-value shells; this series has " + what +
-                        ". With a single shell the radial decay of the signal is not determined by the data: a fit "
-                        "would match the measurements, but its propagator-derived scalars (RTOP, RTAP, RTPP, MSD, "
-                        "QIV) and the signals it predicts at other b-values would reflect the regularization rather "
-                        "than the data. For single-shell data use single-shell three-tissue CSD (cs-ss3t-full, "
-                        "cs_dmri.ss3t_pipeline). To fit anyway for orientation information only, allow single-shell "
-                        "data (--allow-single-shell, or allow_single_shell=True in Python)."};
-            warn.msgs.push_back("the data have " + what +
-                                ": only the orientation information of this fit is meaningful, and "
-                                "propagator-derived scalars are not computed");
-        }
-        const int K = n_coeffs(sp.radial_order);
-        out->n_coeffs = K;
-        if (N < K && sp.regularization != SHOREFIT_REG_L1)
-            warn.msgs.push_back(std::to_string(N) + " measurements for " + std::to_string(K) +
-                                " coefficients (radial order " + std::to_string(sp.radial_order) +
-                                "): without L1 regularization the fit is underdetermined and its fit statistics "
-                                "are not meaningful; use a lower radial order or L1");
-
-        // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
-        // A device mask is compacted on the device; a host mask on the host;
-        // without a mask every voxel is fitted in place (no index list).
-        const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
-        DevBuf<int64_t> d_vox;
-        int64_t V = 0;
-        if (!vol->mask) {
-            V = nvox_total;
-        } else if (vol->on_device) {
-            V = compact_mask(vol->mask, nvox_total, d_vox, s);
-        } else {
-            std::vector<int64_t> vox;
-            vox.reserve(nvox_total);
-            for (int64_t i = 0; i < nvox_total; ++i)
-                if (vol->mask[i]) vox.push_back(i);
-            V = int64_t(vox.size());
-            d_vox.alloc(V);
-            if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
-            SF_HIP(hipStreamSynchronize(s));  // vox is released at scope exit
-        }
-        const int64_t* vox_dev = d_vox.p;  // null: voxel v is row v
-        out->n_voxels = V;
-
-        // --- device buffers -------------------------------------------------
-        const size_t coef_bytes = sizeof(float) * size_t(nvox_total) * K;
-        DevBuf<float> d_data, d_coef, d_r2, d_res, d_alpha, d_bic, d_rssl2;
-        DevBuf<uint32_t> d_it;
-        DevBuf<uint8_t> d_kind;
+        if (std::isnan(big) || std::isnan(small)) tortoise_deltas(grads->bvals, N, &big, &small);
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        const int64_t nvox = vol->nx * vol->ny * vol->nz;
         const bool dev = vol->on_device != 0;
-        const float* data = vol->data;
-        if (!dev) {
-            d_data.alloc(size_t(nvox_total) * N);
-            if (nvox_total)
-                SF_HIP(hipMemcpyAsync(d_data.p, vol->data, sizeof(float) * nvox_total * N, hipMemcpyHostToDevice, s));
-            data = d_data.p;
-        }
-        auto out_buf = [&](auto* user, auto& buf, size_t count) {
-            using T = std::remove_pointer_t<decltype(user)>;
-            if (!user) return static_cast<T*>(nullptr);
-            if (dev) return user;
-            buf.alloc(count);
-            return buf.p;
-        };
-        const bool l1 = sp.regularization == SHOREFIT_REG_L1;
-        float* coef = out_buf(out->coefficients, d_coef, size_t(nvox_total) * K);
-        float* r2 = out_buf(out->r2, d_r2, nvox_total);
-        float* res = out_buf(out->residual_l2, d_res, nvox_total);
-        uint32_t* its = out_buf(out->iterations, d_it, nvox_total);
-        uint8_t* kind = out_buf(out->regularization_kind, d_kind, nvox_total);
-        float* amap = l1 ? out_buf(out->alpha, d_alpha, nvox_total) : nullptr;
-        float* bic = l1 ? out_buf(out->bic, d_bic, nvox_total) : nullptr;
-        const bool path = l1 && (sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC || sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED);
-        const bool anchored = l1 && sp.alpha_mode == SHOREFIT_ALPHA_L2_ANCHORED;
-        // L2-anchored: the Tikhonov reference RSS per voxel (also the rss_l2 map)
-        float* rssl2 = anchored ? out_buf(out->rss_l2, d_rssl2, nvox_total) : nullptr;
-        if (anchored && !rssl2) {
-            d_rssl2.alloc(nvox_total);
-            rssl2 = d_rssl2.p;
-        }
-        DevBuf<float> d_alpha_tmp;
-        const bool per_voxel_alpha = l1 && sp.alpha_mode != SHOREFIT_ALPHA_FIXED;
-        if (per_voxel_alpha && !amap) {
-            d_alpha_tmp.alloc(nvox_total);
-            amap = d_alpha_tmp.p;
-        }
-        // zeros outside the mask (src/fit.rs:89-101)
-        if (nvox_total) {
-            SF_HIP(hipMemsetAsync(coef, 0, coef_bytes, s));
-            if (r2) SF_HIP(hipMemsetAsync(r2, 0, sizeof(float) * nvox_total, s));
-            if (res) SF_HIP(hipMemsetAsync(res, 0, sizeof(float) * nvox_total, s));
-            if (its) SF_HIP(hipMemsetAsync(its, 0, sizeof(uint32_t) * nvox_total, s));
-            if (kind) SF_HIP(hipMemsetAsync(kind, 0, nvox_total, s));
-            if (amap) SF_HIP(hipMemsetAsync(amap, 0, sizeof(float) * nvox_total, s));
-            if (bic) SF_HIP(hipMemsetAsync(bic, 0, sizeof(float) * nvox_total, s));
-        }
-        if (out->rss_l2 && dev && nvox_total) SF_HIP(hipMemsetAsync(out->rss_l2, 0, sizeof(float) * nvox_total, s));
-        if (rssl2 && nvox_total) SF_HIP(hipMemsetAsync(rssl2, 0, sizeof(float) * nvox_total, s));
-
-        // --- design + solver set-up (src/fit.rs:718-759) -------------------
-        const int precision = (l1 && sp.max_iter > 0) ? sp.precision : SHOREFIT_PREC_FP32;
-        Design d;
-        build_design(d, *grads, big, small, sp.radial_order, sp.zeta, s, precision != SHOREFIT_PREC_FP32);
+        DevBuf<uint8_t> d_mask;
+        DevBuf<float> d_data, d_field;
+        const uint8_t* mask = dev_in(vol->mask, size_t(nvox), dev, d_mask, s);
+        const float* data = dev_in(vol->data, size_t(nvox) * N, dev, d_data, s);
+        const float* field = dev_in(gd->field, size_t(nvox) * 9, dev, d_field, s);
+        DevBuf<int64_t> d_vox, d_id, d_dv;
+        const int64_t V = voxel_list(mask, nvox, d_vox, s);
+        int64_t V_dv = 0;
+        gd_partition(field, d_vox.p, V, d_id, d_dv, &V_dv, s);
+        const int64_t n = std::min(V_dv, std::max<int64_t>(max_voxels, 1));
+        if (n == 0) throw Error{SHOREFIT_E_INVALID, "no deviated voxels to tune on"};
+        GdSetup S;
+        gd_setup(S, *grads, big, small, *gd, sp.radial_order, sp.zeta, s);
+        DevBuf<float> coef(size_t(nvox) * S.B.K);
+        GdRunOut o{coef.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         hipEvent_t e0, e1;
         SF_HIP(hipEventCreate(&e0));
         SF_HIP(hipEventCreate(&e1));
-        if (l1) {
-            FistaOps ops;
-            FistaLaunch L{};
-            FistaWork work;
-            L.data = data;
-            L.vox = vox_dev;
-            L.n_vox = V;
-            L.N = d.N;
-            L.sig_stride = N;
-            L.K = d.K;
-            L.Kp = d.Kp;
-            L.Np = d.Np;
-            L.ratio = sp.alpha_mode == SHOREFIT_ALPHA_RATIO ? float(sp.alpha_ratio) : 0.f;
-            L.alpha = sp.alpha;
-            L.max_iter = sp.max_iter;
-            L.tol = float(sp.tol);
-            L.nonneg = sp.non_negative;
-            L.precision = precision;
-            L.refresh = std::max(1, sp.refresh_every);
-            L.coef = coef;
-            L.resid = res;
-            L.r2 = r2;
-            L.iters = its;
-            L.kind = kind;
-            L.alpha_map = amap;
-            L.bic = bic;
-            // alpha path: f_i = exp(t_i ln eps), t_i = i / (n - 1), so that
-            // alpha_i = alpha_max f_i (src/solver/alpha.rs:57-69)
-            DevBuf<float> d_pf;
-            DevBuf<double> d_pf64;
-            TikOps tik;
-            if (path) {
-                double eps = sp.path_eps;
-                if (!(eps > 0.0)) eps = sp.alpha_mode == SHOREFIT_ALPHA_PATH_BIC ? 1e-3 : 1e-4;
-                const int n = int(sp.path_n_alphas);
-                std::vector<double> f64(n);
-                std::vector<float> f32(n);
-                for (int i = 0; i < n; ++i) {
-                    f64[i] = std::exp(double(i) / double(n - 1) * std::log(eps));
-                    f32[i] = float(f64[i]);
-                }
-                d_pf.alloc(n);
-                d_pf64.alloc(n);
-                SF_HIP(hipMemcpyAsync(d_pf.p, f32.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
-                SF_HIP(hipMemcpyAsync(d_pf64.p, f64.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-                SF_HIP(hipStreamSynchronize(s));
-                L.path_n = n;
-                L.path_f = d_pf.p;
-                L.path_f64 = d_pf64.p;
-                L.path_rule = anchored ? 1 : 0;
-                L.slack = sp.slack;
-                L.rss_l2_ref = rssl2;
-            }
-            setup_fista(d, ops, work, L, s);
-            out->lipschitz = d.L;
-            L.thr = float(ops.step * sp.alpha);
-            if (anchored && V > 0) {
-                // the Tikhonov reference fit (src/fit.rs:404-410, 760-761)
-                std::vector<double> dn, dl;
-                regularization_diagonals(sp.radial_order, dn, dl);
-                pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, tik, s);
-                TikhonovLaunch T{};
-                T.data = data;
-                T.vox = vox_dev;
-                T.n_vox = V;
-                T.N = d.N;
-                T.sig_stride = N;
-                T.K = d.K;
-                T.Kp = d.Kp;
-                T.Np = d.Np;
-                T.Pfrag = tik.P.p;
-                T.Afrag = tik.A.p;
-                T.rss_l2 = rssl2;
-                launch_tikhonov(T, s);
-            }
+        double best_ms = 0.0;
+        for (int i = -1; i < nv; ++i) {  // i = -1: warm-up with the default
+            const GdVariant& v = vs[std::max(i, 0)];
             SF_HIP(hipEventRecord(e0, s));
-            L.event_mid = e1;  // recorded right after the solve kernel
-            launch_fista(L, s);
-            if (V == 0) SF_HIP(hipEventRecord(e1, s));
-        } else {
-            d.G.alloc(size_t(d.K) * d.K);
-            launch_gram(d.A.p, d.N, d.K, d.G.p, s);
-            std::vector<double> dn, dl;
-            regularization_diagonals(sp.radial_order, dn, dl);
-            TikOps ops;
-            pack_tikhonov(d, dn.data(), dl.data(), sp.lambda_n, sp.lambda_l, ops, s);
-            TikhonovLaunch L{};
-            L.data = data;
-            L.vox = vox_dev;
-            L.n_vox = V;
-            L.N = d.N;
-            L.sig_stride = N;
-            L.K = d.K;
-            L.Kp = d.Kp;
-            L.Np = d.Np;
-            L.Pfrag = ops.P.p;
-            L.Afrag = ops.A.p;
-            L.coef = coef;
-            L.resid = res;
-            L.r2 = r2;
-            L.iters = its;
-            L.kind = kind;
-            SF_HIP(hipEventRecord(e0, s));
-            launch_tikhonov(L, s);
+            gd_run(S, v, sp, data, N, field, d_dv.p, n, o, nullptr, s);
             SF_HIP(hipEventRecord(e1, s));
+            SF_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            SF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (i < 0) continue;
+            if (ms_out) ms_out[i] = ms;
+            if (best < 0 || ms < best_ms) {
+                best = i;
+                best_ms = ms;
+            }
         }
-        SF_HIP(hipStreamSynchronize(s));
-        float ms = 0.f;
-        SF_HIP(hipEventElapsedTime(&ms, e0, e1));
-        out->fit_ms = ms;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(tuned_mutex());
+        tuned_variants()[{sp.radial_order, N}] = best;
+    });
+    return rc == SHOREFIT_OK ? best : -rc;
+}
 
-        // --- alpha distribution (src/fit.rs:334-352) ------------------------
-        out->has_alpha_distribution = 0;
-        if (per_voxel_alpha) {
-            // the masked voxels' alphas in work order, then three order
-            // statistics (the values a full sort would put at those ranks)
-            std::vector<float> a_v(V);
-            if (V) {
-                DevBuf<float> d_av(V);
-                launch_gather_f32(amap, vox_dev, V, d_av.p, s);
-                SF_HIP(hipMemcpyAsync(a_v.data(), d_av.p, sizeof(float) * V, hipMemcpyDeviceToHost, s));
-                SF_HIP(hipStreamSynchronize(s));
-            }
-            std::vector<double> seen(a_v.begin(), a_v.end());
-            out->has_alpha_distribution = 1;
-            if (seen.empty()) {
-                out->alpha_distribution[0] = out->alpha_distribution[1] = out->alpha_distribution[2] = 0.0;
-            } else {
-                auto pct = [&](double q) {
-                    size_t i = std::min(size_t(std::llround(double(seen.size() - 1) * q)), seen.size() - 1);
-                    std::nth_element(seen.begin(), seen.begin() + i, seen.end());
-                    return seen[i];
-                };
-                out->alpha_distribution[0] = pct(0.5);
-                out->alpha_distribution[1] = pct(0.10);
-                out->alpha_distribution[2] = pct(0.90);
-            }
+int shorefit_graddev_gram(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
+                          uint32_t radial_order, double zeta, const int64_t* voxels, int64_t n_voxels,
+                          float* gram_out, double* rhs_out, double* lipschitz_out) {
+    return guarded([&] {
+        if (!grads || !vol || !gd || !gd->field || (!voxels && n_voxels > 0))
+            throw Error{SHOREFIT_E_INVALID, "null argument"};
+        if (vol->on_device) throw Error{SHOREFIT_E_INVALID, "shorefit_graddev_gram takes host memory"};
+        check_dims(vol->nx, vol->ny, vol->nz);
+        const GdVariant& v = pick_variant(radial_order, grads->n_grads, 0);
+        const int64_t N = grads->n_grads;
+        if (N <= 0 || !grads->bvals || !grads->bvecs) throw Error{SHOREFIT_E_INVALID, "empty gradient table"};
+        const int64_t nvox = vol->nx * vol->ny * vol->nz;
+        for (int64_t i = 0; i < n_voxels; ++i)
+            if (voxels[i] < 0 || voxels[i] >= nvox) throw Error{SHOREFIT_E_INVALID, "voxel index out of range"};
+        if (n_voxels == 0) return;
+        double big = grads->big_delta, small = grads->small_delta;
+        if (std::isnan(big) || std::isnan(small)) tortoise_deltas(grads->bvals, N, &big, &small);
+        hipStream_t s = nullptr;
+        StreamScope scope(s);
+        DevBuf<float> d_data, d_field;
+        const float* data = dev_in(vol->data, size_t(nvox) * N, false, d_data, s);
+        const float* field = dev_in(gd->field, size_t(nvox) * 9, false, d_field, s);
+        DevBuf<int64_t> d_vox(n_voxels);
+        SF_HIP(hipMemcpyAsync(d_vox.p, voxels, sizeof(int64_t) * n_voxels, hipMemcpyHostToDevice, s));
+        GdSetup S;
+        gd_setup(S, *grads, big, small, *gd, radial_order, zeta, s);
+        const int KP = v.kp, K = S.B.K;
+        DevBuf<float> G(size_t(n_voxels) * KP * KP);
+        DevBuf<double> b(size_t(n_voxels) * KP);
+        GdChunk c{S.B, S.tab, data, N, field, d_vox.p, n_voxels, G.p, b.p};
+        gd_launch_gram(v, c, s);
+        std::vector<double> lip(n_voxels);
+        if (lipschitz_out) {
+            // the FISTA kernel's set-up (Lanczos) with no iterations
+            DevBuf<double> d_lip(n_voxels);
+            DevBuf<float> coef(size_t(nvox) * K), av(n_voxels), mom(1);
+            DevBuf<uint32_t> its(n_voxels);
+            DevBuf<uint8_t> st(n_voxels);
+            DevBuf<unsigned> ctr(1);
+            SF_HIP(hipMemsetAsync(ctr.p, 0, sizeof(unsigned), s));
+            GdFistaArgs fa{};
+            fa.G = G.p;
+            fa.b = b.p;
+            fa.vox = d_vox.p;
+            fa.n = n_voxels;
+            fa.N = int(N);
+            fa.K = K;
+            fa.alpha = 1.0;
+            fa.max_iter = 0;
+            fa.tol = 1e-6;
+            fa.mom = mom.p;
+            fa.counter = ctr.p;
+            fa.coef = coef.p;
+            fa.its = its.p;
+            fa.status = st.p;
+            fa.alpha_v = av.p;
+            fa.lip = d_lip.p;
+            gd_launch_fista(v, fa, 0, s);
+            SF_HIP(hipMemcpyAsync(lip.data(), d_lip.p, sizeof(double) * n_voxels, hipMemcpyDeviceToHost, s));
         }
-
-        // --- results back to the caller -------------------------------------
-        if (!dev && nvox_total) {
-            SF_HIP(hipMemcpy(out->coefficients, coef, coef_bytes, hipMemcpyDeviceToHost));
-            if (out->r2) SF_HIP(hipMemcpy(out->r2, r2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
-            if (out->residual_l2)
-                SF_HIP(hipMemcpy(out->residual_l2, res, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
-            if (out->iterations)
-                SF_HIP(hipMemcpy(out->iterations, its, sizeof(uint32_t) * nvox_total, hipMemcpyDeviceToHost));
-            if (out->regularization_kind)
-                SF_HIP(hipMemcpy(out->regularization_kind, kind, nvox_total, hipMemcpyDeviceToHost));
-            if (l1 && out->alpha) SF_HIP(hipMemcpy(out->alpha, amap, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
-            if (l1 && out->bic) SF_HIP(hipMemcpy(out->bic, bic, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
-            if (anchored && out->rss_l2)
-                SF_HIP(hipMemcpy(out->rss_l2, rssl2, sizeof(float) * nvox_total, hipMemcpyDeviceToHost));
+        std::vector<float> Gh(gram_out ? size_t(n_voxels) * KP * KP : 0);
+        std::vector<double> bh(rhs_out ? size_t(n_voxels) * KP : 0);
+        if (gram_out) SF_HIP(hipMemcpyAsync(Gh.data(), G.p, sizeof(float) * Gh.size(), hipMemcpyDeviceToHost, s));
+        if (rhs_out) SF_HIP(hipMemcpyAsync(bh.data(), b.p, sizeof(double) * bh.size(), hipMemcpyDeviceToHost, s));
+        SF_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n_voxels; ++i) {
+            if (gram_out)
+                for (int r = 0; r < K; ++r)
+                    std::memcpy(gram_out + (size_t(i) * K + r) * K, Gh.data() + (size_t(i) * KP + r) * KP,
+                                sizeof(float) * K);
+            if (rhs_out) std::memcpy(rhs_out + size_t(i) * K, bh.data() + size_t(i) * KP, sizeof(double) * K);
+            if (lipschitz_out) lipschitz_out[i] = lip[i];
         }
-        warn.to(out);
     });
 }
 

@@ -206,6 +206,94 @@ struct TikhonovLaunch {
 };
 void launch_tikhonov(const TikhonovLaunch& L, hipStream_t s);
 
+// ------------------------------------------------ gradient-deviation fits ----
+// SHORE basis tables for per-voxel designs (radial order <= 10, K <= 256):
+// column -> (n, l) pair and angular factor; pair -> (n, l, kappa, first
+// column); angular factor -> (l, |m|, kind, normalisation). Angular factors
+// are numbered l (l - 1) / 2 + slot over even l, slots in BrainSuite order.
+struct GdBasis {
+    int R, K, P, Q;
+    double zeta;
+    uint8_t col_pair[256], col_ang[256];
+    uint8_t pair_n[40], pair_l[40];
+    int16_t pair_col0[40];
+    double kappa[40];
+    uint8_t ang_l[72], ang_k[72], ang_kind[72];  // kind 0: l = 0; 1: cos(k theta); 2: m = 0; 3: sin(k theta)
+    double ang_c[72];                           // normalisation (with sqrt 2 and sign)
+};
+GdBasis gd_basis_host(uint32_t radial_order, double zeta);
+
+// Nominal table on the device and the frame of the effective directions.
+struct GdTable {
+    const double* bvals;  // [N] nominal b
+    const double* gvox;   // [N][3] voxel-axis bvecs
+    const double* gnom;   // [N][3] bvecs in the fit frame (kept for b0 / zero rows)
+    double R[9];          // frame rotation, row-major
+    double b0_thr;
+    double qdenom;        // 4 pi^2 tau
+    int N;
+};
+
+// One chunk of deviated voxels (slot i of the chunk is voxel vox[i]).
+struct GdChunk {
+    GdBasis B;
+    GdTable tab;
+    const float* data;
+    int64_t sig_stride;
+    const float* field;   // (X,Y,Z,9) f32 row-major T
+    const int64_t* vox;
+    int64_t n;
+    float* G;             // [n][Kp][Kp]
+    double* b;            // [n][Kp]
+};
+
+struct GdFistaArgs {
+    const float* G;
+    const double* b;
+    const int64_t* vox;
+    int64_t n;
+    int N, K;
+    double alpha;         // fixed alpha
+    double ratio;         // > 0: alpha = ratio * alpha_max (alpha-ratio strategy)
+    uint32_t max_iter;
+    double tol;
+    int nonneg;
+    const float* mom;     // [max_iter] (t_k - 1) / t_{k+1}
+    unsigned* counter;
+    float* coef;          // volume, [.. x K]
+    uint32_t* its;        // [n]
+    uint8_t* status;      // [n] bit 0 converged, bit 1 all-zero short circuit
+    float* alpha_v;       // [n]
+    double* lip;          // [n] optional
+};
+
+struct GdOutArgs {
+    const float* coef;
+    const uint32_t* its;
+    const uint8_t* status;
+    const float* alpha_v;
+    float* resid;
+    float* r2;
+    uint32_t* iters;
+    uint8_t* converged;
+    uint8_t* kind;
+    float* alpha_map;
+    float* bic;
+};
+
+// Tile shapes: padded coefficient count, Gram rows per LDS chunk, Gram
+// waves, FISTA waves (RB = kp / (4 fista_waves) rows per lane group).
+struct GdVariant {
+    int order, kp, rc, gram_waves, fista_waves;
+};
+int gd_variants(uint32_t order, const GdVariant** first);
+void gd_launch_gram(const GdVariant& v, const GdChunk& c, hipStream_t s);
+void gd_launch_fista(const GdVariant& v, const GdFistaArgs& a, int max_blocks, hipStream_t s);
+void gd_launch_residual(const GdChunk& c, const GdOutArgs& o, hipStream_t s);
+void gd_launch_identity(const float* field, const int64_t* vox, int64_t n, uint8_t* ident, uint8_t* devi,
+                        hipStream_t s);
+void launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t n, int64_t* dst, hipStream_t s);
+
 // ------------------------------------------------------------- post-fit ----
 // Per-mode weights of the microstructure scalars for one (radial order,
 // zeta); blocks are the (n, l) pairs in coefficient order.

@@ -1,0 +1,803 @@
+// Per-voxel SHORE fits under a gradient-deviation field.
+//
+// A deviated voxel v has its own gradient table (src/graddev.rs:160-185)
+//     q = T_v^T g_vox / |g_vox|,  b_eff = b |q|^2,  dir = normalize(R q)
+// and therefore its own design A_v (N x K) and Lipschitz constant. The
+// reference pattern for such fits is a per-voxel design build followed by the
+// ordinary solver (src/dti.rs:393-407); here A_v is never stored anywhere but
+// LDS, and the fit runs in Gram form on G_v = A_v^T A_v:
+//
+//   gd_gram_kernel      one workgroup per voxel. Effective table and per-row
+//                       geometry (one thread per row), then RC design rows at
+//                       a time: radial R_nl(q) and angular Y_l^m factors
+//                       (one item per thread), their products into an FP64
+//                       row block in LDS, b_v += A^T s (FP64) and the upper
+//                       16x16 tiles of G_v += A^T A on the FP64 MFMA. G_v is
+//                       written as FP32 (Kp x Kp, both triangles) and b_v as
+//                       FP64 to a workspace.
+//   gd_fista_kernel     persistent; a workgroup claims voxels from a counter.
+//                       Lane (row group rb, column group cb) keeps the RB x CB
+//                       block of G_v in registers. Lanczos (LANCZOS_STEPS
+//                       steps, FP64 dot products) gives sigma_max(A_v)^2, then
+//                       FISTA (src/solver/fista.rs:70-143) in Gram form
+//                           v = y + b/(L N) - (G y)/(L N)
+//                       with one workgroup barrier per iteration.
+//   gd_residual_kernel  one workgroup per voxel: the rows again, the FP64
+//                       residual |s - A_v x|, R^2, BIC and the maps.
+//
+// FP64 is used where the shared-design path uses it: design, Gram matrix,
+// right-hand side, Lipschitz estimate, residual. The iteration itself is FP32.
+#include "common.h"
+#include "special.h"
+
+namespace shorefit {
+
+// ------------------------------------------------------------ host tables ----
+
+namespace {
+double fact_h(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= double(i);
+    return f;
+}
+double gamma_n32_h(int n) {
+    double g = std::sqrt(kPiD) * 0.5;
+    for (int i = 1; i <= n; ++i) g *= (double(i) + 0.5);
+    return g;
+}
+}  // namespace
+
+GdBasis gd_basis_host(uint32_t R, double zeta) {
+    GdBasis b{};
+    b.R = int(R);
+    b.zeta = zeta;
+    int col = 0, p = 0;
+    for (int n = 0; n <= int(R); ++n)
+        for (int l = 0; l <= n; l += 2) {
+            b.pair_n[p] = uint8_t(n);
+            b.pair_l[p] = uint8_t(l);
+            b.pair_col0[p] = int16_t(col);
+            // src/basis/shore.rs:56-61
+            b.kappa[p] = std::sqrt(2.0 * fact_h(n - l) / (std::pow(zeta, 1.5) * gamma_n32_h(n)));
+            for (int slot = 0; slot <= 2 * l; ++slot) {
+                b.col_pair[col] = uint8_t(p);
+                b.col_ang[col] = uint8_t(l * (l - 1) / 2 + slot);
+                ++col;
+            }
+            ++p;
+        }
+    b.K = col;
+    b.P = p;
+    int a = 0;
+    for (int l = 0; l <= int(R); l += 2)
+        for (int slot = 0; slot <= 2 * l; ++slot, ++a) {
+            // src/basis/shore.rs:76-115 (BrainSuite block)
+            b.ang_l[a] = uint8_t(l);
+            if (l == 0) {
+                b.ang_kind[a] = 0;
+                b.ang_k[a] = 0;
+                b.ang_c[a] = 1.0 / std::sqrt(4.0 * kPiD);
+                continue;
+            }
+            const int k = slot < l ? l - slot : slot - l;
+            const double norm = std::sqrt((2.0 * l + 1.0) / (4.0 * kPiD) * fact_h(l - k) / fact_h(l + k));
+            b.ang_k[a] = uint8_t(k);
+            if (slot < l) {
+                b.ang_kind[a] = 1;
+                b.ang_c[a] = std::sqrt(2.0) * norm;
+            } else if (slot == l) {
+                b.ang_kind[a] = 2;
+                b.ang_c[a] = norm;
+            } else {
+                b.ang_kind[a] = 3;
+                b.ang_c[a] = std::sqrt(2.0) * ((k % 2 == 1) ? 1.0 : -1.0) * norm;
+            }
+        }
+    b.Q = a;
+    return b;
+}
+
+namespace {
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLanczos = 24;  // Lanczos steps for sigma_max^2 (~1e-8 relative)
+
+constexpr int n_pairs_c(int R) {
+    int p = 0;
+    for (int n = 0; n <= R; ++n) p += n / 2 + 1;
+    return p;
+}
+constexpr int n_coeffs_c(int R) {
+    int k = 0;
+    for (int n = 0; n <= R; ++n)
+        for (int l = 0; l <= n; l += 2) k += 2 * l + 1;
+    return k;
+}
+constexpr int n_ang_c(int R) { return (R + 1) * (R + 2) / 2; }  // sum over even l <= R of 2l+1
+struct RowGeom {
+    double u, e, cp, ct, st;
+};
+
+// Effective table row i of deviation T (row-major, g_eff = T^T g): b and
+// direction (src/graddev.rs:160-185), q-value (src/qspace.rs:130-149), and
+// the spherical coordinates of the design (src/basis/shore.rs:129-138,
+// src/math.rs:157-166): u = q^2 / zeta, e = exp(-u / 2), cos(phi), cos and
+// sin of theta.
+__device__ inline RowGeom gd_row(const GdTable& t, const double (&T)[9], int i, double zeta) {
+    const double b = t.bvals[i];
+    const double g0 = t.gvox[3 * i], g1 = t.gvox[3 * i + 1], g2 = t.gvox[3 * i + 2];
+    const double gn = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    double be, d0, d1, d2;
+    if (!(b > t.b0_thr) || !(gn > 1e-12)) {
+        be = b;
+        d0 = t.gnom[3 * i];
+        d1 = t.gnom[3 * i + 1];
+        d2 = t.gnom[3 * i + 2];
+    } else {
+        const double u0 = g0 / gn, u1 = g1 / gn, u2 = g2 / gn;
+        const double q0 = T[0] * u0 + T[3] * u1 + T[6] * u2;
+        const double q1 = T[1] * u0 + T[4] * u1 + T[7] * u2;
+        const double q2 = T[2] * u0 + T[5] * u1 + T[8] * u2;
+        const double e0 = t.R[0] * q0 + t.R[1] * q1 + t.R[2] * q2;
+        const double e1 = t.R[3] * q0 + t.R[4] * q1 + t.R[5] * q2;
+        const double e2 = t.R[6] * q0 + t.R[7] * q1 + t.R[8] * q2;
+        const double dn = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+        be = b * (q0 * q0 + q1 * q1 + q2 * q2);
+        d0 = e0 / dn;
+        d1 = e1 / dn;
+        d2 = e2 / dn;
+    }
+    const double qv = (be <= t.b0_thr) ? 0.0 : sqrt(be / t.qdenom);
+    const double x = d0 * qv, y = d1 * qv, z = d2 * qv;
+    const double r = sqrt(x * x + y * y + z * z);
+    double th = atan2(y, x);
+    if (isnan(th)) th = 0.0;
+    const double ph = r > 0.0 ? acos(fmin(1.0, fmax(-1.0, z / r))) : 0.0;
+    RowGeom g;
+    g.u = r * r / zeta;
+    g.e = exp(-g.u * 0.5);
+    g.cp = cos(ph);
+    sincos(th, &g.st, &g.ct);
+    return g;
+}
+
+__device__ inline double upow_half_l(double u, int l) {
+    double p = 1.0;
+    for (int j = 0; j < l / 2; ++j) p *= u;
+    return p;
+}
+
+// Radial factor of pair p (src/basis/shore.rs:64-70).
+__device__ inline double gd_radial(const GdBasis& B, int p, double u, double e) {
+    const int n = B.pair_n[p], l = B.pair_l[p];
+    return B.kappa[p] * laguerre_d(n - l, l + 0.5, u) * e * upow_half_l(u, l);
+}
+
+// Angular factor a (BrainSuite order, src/basis/shore.rs:76-115); cos and
+// sin of k theta by angle addition from cos and sin of theta.
+__device__ inline double gd_angular(const GdBasis& B, int a, double cp, double ct, double st) {
+    const int kind = B.ang_kind[a];
+    if (kind == 0) return B.ang_c[a];
+    const int l = B.ang_l[a], k = B.ang_k[a];
+    const double p = legendre_d(l, k, cp);
+    if (kind == 2) return B.ang_c[a] * p;
+    double ck = 1.0, sk = 0.0;
+    for (int j = 0; j < k; ++j) {
+        const double cn = ck * ct - sk * st;
+        sk = sk * ct + ck * st;
+        ck = cn;
+    }
+    return B.ang_c[a] * p * (kind == 1 ? ck : sk);
+}
+
+__device__ inline void load_T(const float* field, int64_t v, double (&T)[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) T[k] = double(field[v * 9 + k]);
+}
+
+// ------------------------------------------------------------ Gram kernel ----
+template <int R, int KP, int RC, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES, 2)
+    gd_gram_kernel(const GdBasis B, const GdTable tab, const float* __restrict__ data, int64_t sig_stride,
+                   const float* __restrict__ field, const int64_t* __restrict__ vox, int64_t n,
+                   float* __restrict__ Gws, double* __restrict__ bws) {
+    constexpr int K = n_coeffs_c(R), P = n_pairs_c(R), Q = n_ang_c(R), PQ = P + Q;
+    constexpr int KT = KP / 16, NT = KT * (KT + 1) / 2, TPW = (NT + WAVES - 1) / WAVES;
+    constexpr int LDA = KP + 2;
+    constexpr int NTH = 64 * WAVES;
+    static_assert(KP % 16 == 0 && KP >= K && RC % 4 == 0 && KP <= NTH, "tile shape");
+    const int64_t slot = blockIdx.x;
+    if (slot >= n) return;
+    const int64_t v = vox[slot];
+    const int N = tab.N;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    extern __shared__ double smem[];
+    double* s_u = smem;
+    double* s_e = s_u + N;
+    double* s_cp = s_e + N;
+    double* s_ct = s_cp + N;
+    double* s_st = s_ct + N;
+    double* s_sig = s_st + N;
+    double* s_fac = s_sig + N;     // RC x PQ
+    double* s_A = s_fac + RC * PQ;  // RC x LDA
+
+    double T[9];
+    load_T(field, v, T);
+    for (int i = tid; i < N; i += NTH) {
+        const RowGeom g = gd_row(tab, T, i, B.zeta);
+        s_u[i] = g.u;
+        s_e[i] = g.e;
+        s_cp[i] = g.cp;
+        s_ct[i] = g.ct;
+        s_st[i] = g.st;
+        s_sig[i] = double(data[v * sig_stride + i]);
+    }
+
+    // tiles of this wave: t = w + WAVES j in row-major order of the upper
+    // triangle (I <= J)
+    int tI[TPW], tJ[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        int t = w + WAVES * j, I = 0;
+        while (I < KT && t >= KT - I) {
+            t -= KT - I;
+            ++I;
+        }
+        tI[j] = I;
+        tJ[j] = I + t;
+    }
+    doublex4 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    double bacc = 0.0;
+
+    for (int i0 = 0; i0 < N; i0 += RC) {
+        __syncthreads();  // geometry ready; previous chunk's A reads done
+        for (int it = tid; it < RC * PQ; it += NTH) {
+            const int r = it / PQ, f = it - r * PQ;
+            const int i = i0 + r;
+            double val = 0.0;
+            if (i < N) val = f < P ? gd_radial(B, f, s_u[i], s_e[i]) : gd_angular(B, f - P, s_cp[i], s_ct[i], s_st[i]);
+            s_fac[it] = val;
+        }
+        __syncthreads();
+        for (int it = tid; it < RC * KP; it += NTH) {
+            const int r = it / KP, c = it - r * KP;
+            double val = 0.0;
+            if (c < K) val = s_fac[r * PQ + B.col_pair[c]] * s_fac[r * PQ + P + B.col_ang[c]];
+            s_A[r * LDA + c] = val;
+        }
+        __syncthreads();
+        if (tid < KP) {
+            const int rmax = min(RC, N - i0);
+            for (int r = 0; r < rmax; ++r) bacc += s_A[r * LDA + tid] * s_sig[i0 + r];
+        }
+#pragma unroll
+        for (int kk = 0; kk < RC / 4; ++kk) {
+            const double* rowp = s_A + (4 * kk + (lane >> 4)) * LDA + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) {
+                if (w + WAVES * j < NT) {
+                    const double a = rowp[16 * tI[j]], b = rowp[16 * tJ[j]];
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    // D layout of the 16x16x4 f64 MFMA (measured on gfx950; it differs from
+    // the f32 variant's): lane l, register r holds row 4 r + l / 16, column
+    // l % 16 of the tile.
+    float* Gv = Gws + size_t(slot) * KP * KP;
+    const int rq = lane >> 4, cl = lane & 15;
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        if (w + WAVES * j >= NT) continue;
+        const int I = tI[j], J = tJ[j];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float gv = float(acc[j][r]);
+            Gv[size_t(16 * I + 4 * r + rq) * KP + 16 * J + cl] = gv;
+            if (I != J) Gv[size_t(16 * J + cl) * KP + 16 * I + 4 * r + rq] = gv;
+        }
+    }
+    if (tid < KP) bws[size_t(slot) * KP + tid] = bacc;
+}
+
+// ------------------------------------------------------------ FISTA kernel ----
+
+// Sum over the 16 lanes of a row (DPP): every lane ends with the same value.
+__device__ __forceinline__ float row16_sum(float x) {
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));  // quad [1,0,3,2]
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));  // quad [2,3,0,1]
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false)); // half mirror
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false)); // row mirror
+    return x;
+}
+
+// Wave sum of a value that is uniform within each row of 16 lanes.
+__device__ __forceinline__ float rows4_sum(float x) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
+}
+
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x += __shfl_xor(x, m);
+    return x;
+}
+__device__ __forceinline__ double wave_max_d(double x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x = fmax(x, __shfl_xor(x, m));
+    return x;
+}
+
+template <int KP, int W>
+__global__ void __launch_bounds__(64 * W) gd_fista_kernel(const GdFistaArgs a) {
+    constexpr int RB = KP / (4 * W), CB = KP / 16;
+    static_assert(RB * 4 * W == KP && CB * 16 == KP && RB <= 16, "block shape");
+    __shared__ float s_vec[2][KP];
+    __shared__ double s_red[2][W][2];
+    __shared__ double s_al[kLanczos], s_be[kLanczos];
+    __shared__ int64_t s_slot;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int rb = 4 * w + (lane >> 4), cb = lane & 15;
+    const int r0 = rb * RB, c0 = cb * CB;
+    const bool owner = cb < RB;
+    const int j = r0 + cb;  // the row this lane owns (owner lanes)
+    const int N = a.N, K = a.K;
+    int rp = 0;             // reduction buffer parity
+
+    // Block sums / max of FP64 values (one barrier each).
+    auto block_sum2 = [&](double x, double y, double& sx, double& sy) {
+        x = wave_sum_d(x);
+        y = wave_sum_d(y);
+        if (lane == 0) {
+            s_red[rp][w][0] = x;
+            s_red[rp][w][1] = y;
+        }
+        __syncthreads();
+        sx = 0.0;
+        sy = 0.0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            sx += s_red[rp][q][0];
+            sy += s_red[rp][q][1];
+        }
+        rp ^= 1;
+    };
+    // (G v)_j for owner lanes, v = s_vec[buf]
+    auto matvec = [&](const float (&g)[RB][CB], int buf) -> float {
+        float vv[CB];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) vv[c] = s_vec[buf][c0 + c];
+        float accr[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            float s = 0.f;
+#pragma unroll
+            for (int c = 0; c < CB; ++c) s = fmaf(g[r][c], vv[c], s);
+            accr[r] = row16_sum(s);
+        }
+        float mine = accr[0];
+#pragma unroll
+        for (int r = 1; r < RB; ++r) mine = (cb == r) ? accr[r] : mine;
+        return mine;
+    };
+
+    for (;;) {
+        if (tid == 0) s_slot = int64_t(atomicAdd(a.counter, 1u));
+        __syncthreads();
+        const int64_t slot = s_slot;
+        if (slot >= a.n) break;
+
+        float g[RB][CB];
+        {
+            const float* Gp = a.G + size_t(slot) * KP * KP + size_t(r0) * KP + c0;
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) g[r][c] = Gp[size_t(r) * KP + c];
+        }
+        const double bj = (owner && j < K) ? a.b[size_t(slot) * KP + j] : 0.0;
+
+        // --- Lanczos on G_v for sigma_max(A_v)^2 ------------------------------
+        // deterministic start vector with every component non-zero
+        const double q0 = (owner && j < K) ? 1.0 + 0.5 * double((j * 37) % 11) / 11.0 : 0.0;
+        double qn2, amax, dummy;
+        block_sum2(q0 * q0, 0.0, qn2, dummy);
+        {
+            const double mx = wave_max_d(fabs(bj));
+            if (lane == 0) s_red[rp][w][0] = mx;
+            __syncthreads();
+            amax = 0.0;
+#pragma unroll
+            for (int q = 0; q < W; ++q) amax = fmax(amax, s_red[rp][q][0]);
+            rp ^= 1;
+        }
+        double qv = q0 / sqrt(qn2);
+        double qprev = 0.0, beta_prev = 0.0;
+        int m = 0;
+        for (; m < kLanczos && m < K; ++m) {
+            if (owner) s_vec[m & 1][j] = float(qv);
+            __syncthreads();
+            const double gq = double(matvec(g, m & 1));
+            double wv = owner ? gq - beta_prev * qprev : 0.0;
+            double al;
+            block_sum2(wv * qv, 0.0, al, dummy);
+            wv -= al * qv;
+            double bb;
+            block_sum2(wv * wv, 0.0, bb, dummy);
+            const double be = sqrt(bb);
+            if (tid == 0) {
+                s_al[m] = al;
+                s_be[m] = be;
+            }
+            qprev = qv;
+            beta_prev = be;
+            if (!(be > 1e-30 * fabs(al))) {
+                ++m;
+                break;
+            }
+            qv = wv / be;
+        }
+        __syncthreads();
+        // largest eigenvalue of the m x m tridiagonal: Sturm counts at 64
+        // points per round (one per lane), 7 rounds of 65x narrowing
+        double lam;
+        {
+            double lo = 0.0, hi = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double off = (i > 0 ? fabs(s_be[i - 1]) : 0.0) + (i + 1 < m ? fabs(s_be[i]) : 0.0);
+                lo = i == 0 ? s_al[i] - off : fmin(lo, s_al[i] - off);
+                hi = i == 0 ? s_al[i] + off : fmax(hi, s_al[i] + off);
+            }
+            for (int round = 0; round < 7; ++round) {
+                const double x = lo + (hi - lo) * double(lane + 1) / 65.0;
+                int cnt = 0;
+                double d = 1.0;
+                for (int i = 0; i < m; ++i) {
+                    const double b2 = i > 0 ? s_be[i - 1] * s_be[i - 1] : 0.0;
+                    d = s_al[i] - x - (i > 0 ? b2 / d : 0.0);
+                    if (d == 0.0) d = -1e-300;
+                    cnt += d < 0.0;
+                }
+                const uint64_t above = __ballot(cnt == m);  // lanes with x > lambda_max
+                const int f = above ? __builtin_ctzll(above) : 64;
+                const double nlo = f == 0 ? lo : lo + (hi - lo) * double(f) / 65.0;
+                const double nhi = f == 64 ? hi : lo + (hi - lo) * double(f + 1) / 65.0;
+                lo = nlo;
+                hi = nhi;
+            }
+            lam = 0.5 * (lo + hi);
+        }
+        // src/solver/fista.rs:51,58: L = max(sigma_max^2 / N, eps)
+        const double L = fmax(fmax(lam, 0.0) / double(N), 2.220446049250313e-16);
+        const double step = 1.0 / L;
+        double alpha_v = a.alpha;
+        bool zero_voxel = false;
+        if (a.ratio > 0.0) {  // src/solver/alpha.rs:47-53, :178-189
+            const double amx = amax / double(N);
+            alpha_v = amx * a.ratio;
+            zero_voxel = amx == 0.0;
+        }
+        const float c = float(step / double(N));
+        const float bs = float(bj * (step / double(N)));
+        const float thr = float(step * alpha_v);
+
+        // --- FISTA ----------------------------------------------------------
+        float x = 0.f, y = 0.f;
+        uint32_t its = 0;
+        bool conv = false;
+        if (zero_voxel) {
+            conv = true;
+        } else {
+            if (owner) s_vec[0][j] = 0.f;
+            __syncthreads();
+            int buf = 0;
+            for (uint32_t it = 0; it < a.max_iter; ++it) {
+                const float gy = matvec(g, buf);
+                float d2 = 0.f, n2 = 0.f;
+                if (owner) {
+                    const float vj = fmaf(-c, gy, y + bs);
+                    float xn = vj > thr ? vj - thr : (vj < -thr ? vj + thr : 0.f);
+                    if (a.nonneg) xn = fmaxf(xn, 0.f);
+                    const float dx = xn - x;
+                    const float yn = fmaf(a.mom[it], dx, xn);
+                    d2 = dx * dx;
+                    n2 = xn * xn;
+                    x = xn;
+                    y = yn;
+                    s_vec[buf ^ 1][j] = yn;
+                }
+                d2 = rows4_sum(row16_sum(d2));
+                n2 = rows4_sum(row16_sum(n2));
+                if (lane == 0) {
+                    s_red[rp][w][0] = d2;
+                    s_red[rp][w][1] = n2;
+                }
+                __syncthreads();
+                double D2 = 0.0, N2 = 0.0;
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    D2 += s_red[rp][q][0];
+                    N2 += s_red[rp][q][1];
+                }
+                rp ^= 1;
+                buf ^= 1;
+                its = it + 1;
+                // src/solver/fista.rs:113-127: |dx| / max(|x+|, 1e-12) < tol
+                if (sqrt(D2) / fmax(sqrt(N2), 1e-12) < a.tol) {
+                    conv = true;
+                    break;
+                }
+            }
+        }
+        if (owner && j < K) a.coef[a.vox[slot] * K + j] = x;
+        if (tid == 0) {
+            a.its[slot] = its;
+            a.status[slot] = uint8_t((conv ? 1 : 0) | (zero_voxel ? 2 : 0));
+            a.alpha_v[slot] = float(alpha_v);
+            if (a.lip) a.lip[slot] = L;
+        }
+        __syncthreads();  // s_slot / s_vec reuse
+    }
+}
+
+// --------------------------------------------------------- residual kernel ----
+// One workgroup (4 waves) per voxel: the design rows again, RC at a time, as
+// radial x angular factor tables in LDS (the Gram kernel's items); wave w
+// forms rows w, w+4, ... of A_v x with the lanes over the columns.
+template <int R, int RC>
+__global__ void __launch_bounds__(256)
+    gd_residual_kernel(const GdBasis B, const GdTable tab, const float* __restrict__ data, int64_t sig_stride,
+                       const float* __restrict__ field, const int64_t* __restrict__ vox, int64_t n,
+                       const GdOutArgs o) {
+    constexpr int K = n_coeffs_c(R), P = n_pairs_c(R), Q = n_ang_c(R), PQ = P + Q;
+    const int64_t slot = blockIdx.x;
+    if (slot >= n) return;
+    const int64_t v = vox[slot];
+    const int N = tab.N;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    extern __shared__ double smem[];
+    double* s_u = smem;
+    double* s_e = s_u + N;
+    double* s_cp = s_e + N;
+    double* s_ct = s_cp + N;
+    double* s_st = s_ct + N;
+    double* s_sig = s_st + N;
+    double* s_fac = s_sig + N;  // RC x PQ
+    __shared__ double s_x[K];
+    __shared__ double s_red[4][3];
+
+    int nnz = 0;
+    for (int c = tid; c < K; c += 256) {
+        const float xc = o.coef[v * K + c];
+        s_x[c] = double(xc);
+        nnz += xc != 0.f;
+    }
+    double T[9];
+    load_T(field, v, T);
+    double ssum = 0.0;
+    for (int i = tid; i < N; i += 256) {
+        const RowGeom g = gd_row(tab, T, i, B.zeta);
+        s_u[i] = g.u;
+        s_e[i] = g.e;
+        s_cp[i] = g.cp;
+        s_ct[i] = g.ct;
+        s_st[i] = g.st;
+        const double sv = double(data[v * sig_stride + i]);
+        s_sig[i] = sv;
+        ssum += sv;
+    }
+    ssum = wave_sum_d(ssum);
+    const double nz = wave_sum_d(double(nnz));
+    if (lane == 0) {
+        s_red[w][0] = ssum;
+        s_red[w][1] = nz;
+    }
+    __syncthreads();
+    const double mean = (s_red[0][0] + s_red[1][0] + s_red[2][0] + s_red[3][0]) / double(N);
+    const int nnz_all = int(s_red[0][1] + s_red[1][1] + s_red[2][1] + s_red[3][1]);
+
+    double rss = 0.0, sst = 0.0;  // lane 0 of each wave
+    for (int i0 = 0; i0 < N; i0 += RC) {
+        __syncthreads();
+        for (int it = tid; it < RC * PQ; it += 256) {
+            const int r = it / PQ, f = it - r * PQ;
+            const int i = i0 + r;
+            double val = 0.0;
+            if (i < N) val = f < P ? gd_radial(B, f, s_u[i], s_e[i]) : gd_angular(B, f - P, s_cp[i], s_ct[i], s_st[i]);
+            s_fac[it] = val;
+        }
+        __syncthreads();
+        const int rmax = min(RC, N - i0);
+        for (int r = w; r < rmax; r += 4) {
+            double acc = 0.0;
+            for (int c = lane; c < K; c += 64)
+                acc += s_fac[r * PQ + B.col_pair[c]] * s_fac[r * PQ + P + B.col_ang[c]] * s_x[c];
+            acc = wave_sum_d(acc);
+            const double sv = s_sig[i0 + r];
+            const double e = sv - acc;
+            rss += e * e;
+            sst += (sv - mean) * (sv - mean);
+        }
+    }
+    if (lane == 0) {
+        s_red[w][0] = rss;
+        s_red[w][1] = sst;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    rss = s_red[0][0] + s_red[1][0] + s_red[2][0] + s_red[3][0];
+    sst = s_red[0][1] + s_red[1][1] + s_red[2][1] + s_red[3][1];
+    const uint8_t st = o.status[slot];
+    const bool zero_voxel = (st & 2) != 0;
+    // src/fit.rs:120-128 (R^2), src/solver/alpha.rs:99-103 (BIC), :178-189
+    const double ss_res = zero_voxel ? 0.0 : rss;
+    const double resid_l2 = sqrt(ss_res);
+    const double r2 = sst > 0.0 ? 1.0 - ss_res / sst : 0.0;
+    if (o.resid) o.resid[v] = float(resid_l2);
+    if (o.r2) o.r2[v] = float(r2);
+    if (o.iters) o.iters[v] = o.its[slot];
+    if (o.converged) o.converged[v] = st & 1;
+    if (o.kind) o.kind[v] = 1;
+    if (o.alpha_map) o.alpha_map[v] = o.alpha_v[slot];
+    if (o.bic) {
+        const double nn = double(N);
+        const double rc = fmax(resid_l2 * resid_l2, 2.2250738585072014e-308);
+        o.bic[v] = zero_voxel ? 0.f : float(nn * log(rc / nn) + log(nn) * double(nnz_all));
+    }
+}
+
+// identity flags: 1 when max |T - I| < IDENTITY_TOL (src/graddev.rs:35-36, :89-93)
+__global__ void gd_identity_kernel(const float* __restrict__ field, const int64_t* __restrict__ vox, int64_t n,
+                                   uint8_t* __restrict__ ident, uint8_t* __restrict__ devi) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t v = vox ? vox[i] : i;
+    double dev = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dev = fmax(dev, fabs(double(field[v * 9 + k]) - ((k % 4 == 0) ? 1.0 : 0.0)));
+    const bool id = dev < 1e-6;
+    ident[i] = id;
+    devi[i] = !id;
+}
+
+__global__ void gather_i64_kernel(const int64_t* __restrict__ src, const int64_t* __restrict__ pos, int64_t n,
+                                  int64_t* __restrict__ dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src ? src[pos[i]] : pos[i];
+}
+
+template <int R, int KP, int RC, int WAVES>
+size_t gram_lds(int N) {
+    constexpr int P = n_pairs_c(R), Q = n_ang_c(R);
+    return sizeof(double) * (size_t(6) * N + size_t(RC) * (P + Q) + size_t(RC) * (KP + 2));
+}
+
+template <int R, int KP, int RC, int WAVES>
+void launch_gram_t(const GdChunk& c, hipStream_t s) {
+    const size_t lds = gram_lds<R, KP, RC, WAVES>(c.tab.N);
+    if (lds > 160 * 1024) throw Error{SHOREFIT_E_UNSUPPORTED, "gradient-deviation fit: too many gradients for LDS"};
+    auto kern = gd_gram_kernel<R, KP, RC, WAVES>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(lds));
+    hipLaunchKernelGGL(kern, dim3(unsigned(c.n)), dim3(64 * WAVES), lds, s, c.B, c.tab, c.data, c.sig_stride,
+                       c.field, c.vox, c.n, c.G, c.b);
+    SF_HIP(hipGetLastError());
+}
+
+template <int KP, int W>
+void launch_fista_t(const GdFistaArgs& a, int max_blocks, hipStream_t s) {
+    auto kern = gd_fista_kernel<KP, W>;
+    int occ = 0;
+    SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * W, 0));
+    int dev = 0, cus = 0;
+    SF_HIP(hipGetDevice(&dev));
+    SF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int64_t blocks = int64_t(std::max(occ, 1)) * cus;
+    if (max_blocks > 0) blocks = std::min<int64_t>(blocks, max_blocks);
+    blocks = std::min<int64_t>(blocks, std::max<int64_t>(a.n, 1));
+    hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(64 * W), 0, s, a);
+    SF_HIP(hipGetLastError());
+}
+
+template <int R>
+void launch_resid_t(const GdChunk& c, const GdOutArgs& o, hipStream_t s) {
+    constexpr int RC = 32, PQ = n_pairs_c(R) + n_ang_c(R);
+    const size_t lds = sizeof(double) * (size_t(6) * c.tab.N + size_t(RC) * PQ);
+    if (lds > 128 * 1024) throw Error{SHOREFIT_E_UNSUPPORTED, "gradient-deviation fit: too many gradients for LDS"};
+    auto kern = gd_residual_kernel<R, RC>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(lds));
+    hipLaunchKernelGGL(kern, dim3(unsigned(c.n)), dim3(256), lds, s, c.B, c.tab, c.data, c.sig_stride, c.field,
+                       c.vox, c.n, o);
+    SF_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// Variants: (order, Kp, gram rows per chunk, gram waves, FISTA waves).
+static const GdVariant kVariants[] = {
+    {2, 16, 32, 1, 1},   {4, 32, 32, 1, 1},   {4, 32, 32, 1, 2},   {6, 80, 32, 4, 2},
+    {6, 80, 32, 4, 4},   {6, 80, 16, 4, 4},   {8, 160, 32, 4, 4},  {8, 160, 32, 4, 5},
+    {8, 160, 32, 4, 8},  {8, 160, 16, 4, 5},  {10, 256, 16, 8, 8},
+};
+
+int gd_variants(uint32_t order, const GdVariant** out) {
+    int first = -1, count = 0;
+    for (int i = 0; i < int(sizeof(kVariants) / sizeof(kVariants[0])); ++i)
+        if (kVariants[i].order == int(order)) {
+            if (first < 0) first = i;
+            ++count;
+        }
+    *out = first >= 0 ? &kVariants[first] : nullptr;
+    return count;
+}
+
+void gd_launch_gram(const GdVariant& v, const GdChunk& c, hipStream_t s) {
+    if (c.n == 0) return;
+#define SF_GRAM(O, KP, RC, WV) \
+    if (v.order == O && v.kp == KP && v.rc == RC && v.gram_waves == WV) return launch_gram_t<O, KP, RC, WV>(c, s);
+    SF_GRAM(2, 16, 32, 1)
+    SF_GRAM(4, 32, 32, 1)
+    SF_GRAM(6, 80, 32, 4)
+    SF_GRAM(6, 80, 16, 4)
+    SF_GRAM(8, 160, 32, 4)
+    SF_GRAM(8, 160, 16, 4)
+    SF_GRAM(10, 256, 16, 8)
+#undef SF_GRAM
+    throw Error{SHOREFIT_E_UNSUPPORTED, "no Gram kernel for this variant"};
+}
+
+void gd_launch_fista(const GdVariant& v, const GdFistaArgs& a, int max_blocks, hipStream_t s) {
+    if (a.n == 0) return;
+#define SF_FI(KP, W) \
+    if (v.kp == KP && v.fista_waves == W) return launch_fista_t<KP, W>(a, max_blocks, s);
+    SF_FI(16, 1)
+    SF_FI(32, 1)
+    SF_FI(32, 2)
+    SF_FI(80, 2)
+    SF_FI(80, 4)
+    SF_FI(160, 4)
+    SF_FI(160, 5)
+    SF_FI(160, 8)
+    SF_FI(256, 8)
+#undef SF_FI
+    throw Error{SHOREFIT_E_UNSUPPORTED, "no FISTA kernel for this variant"};
+}
+
+void gd_launch_residual(const GdChunk& c, const GdOutArgs& o, hipStream_t s) {
+    if (c.n == 0) return;
+    switch (c.B.R) {
+        case 2: return launch_resid_t<2>(c, o, s);
+        case 4: return launch_resid_t<4>(c, o, s);
+        case 6: return launch_resid_t<6>(c, o, s);
+        case 8: return launch_resid_t<8>(c, o, s);
+        case 10: return launch_resid_t<10>(c, o, s);
+        default: throw Error{SHOREFIT_E_UNSUPPORTED, "no residual kernel for this radial order"};
+    }
+}
+
+void gd_launch_identity(const float* field, const int64_t* vox, int64_t n, uint8_t* ident, uint8_t* devi,
+                        hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(gd_identity_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, field, vox, n, ident,
+                       devi);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t n, int64_t* dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(gather_i64_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, src, pos, n, dst);
+    SF_HIP(hipGetLastError());
+}
+
+}  // namespace shorefit

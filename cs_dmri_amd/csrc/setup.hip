@@ -5,6 +5,7 @@
 #include <hip/hip_fp16.h>
 
 #include "common.h"
+#include "special.h"
 
 namespace shorefit {
 
@@ -18,56 +19,7 @@ int n_coeffs(uint32_t radial_order) {
 
 namespace {
 
-constexpr double kPi = 3.14159265358979323846264338327950288;
-
-__device__ double factorial_d(int n) {
-    double f = 1.0;
-    for (int i = 2; i <= n; ++i) f *= double(i);
-    return f;
-}
-
-// Gamma(n + 3/2) = (2n+1)!! / 2^(n+1) * sqrt(pi), exact product form.
-__device__ double gamma_n_three_halves(int n) {
-    double g = sqrt(kPi) * 0.5;  // Gamma(3/2)
-    for (int i = 1; i <= n; ++i) g *= (double(i) + 0.5);
-    return g;
-}
-
-// Generalised Laguerre L_n^a(x) by its three-term recurrence.
-__device__ double laguerre_d(int n, double a, double x) {
-    double lm1 = 1.0;
-    if (n == 0) return lm1;
-    double l = 1.0 + a - x;
-    for (int k = 1; k < n; ++k) {
-        const double lp1 = ((2.0 * k + 1.0 + a - x) * l - (k + a) * lm1) / (k + 1.0);
-        lm1 = l;
-        l = lp1;
-    }
-    return l;
-}
-
-// Associated Legendre P_l^m(x), Condon-Shortley phase, upward recurrence.
-__device__ double legendre_d(int l, int m, double x) {
-    double pmm = 1.0;
-    if (m > 0) {
-        const double s = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
-        double f = 1.0;
-        for (int i = 0; i < m; ++i) {
-            pmm *= -f * s;
-            f += 2.0;
-        }
-    }
-    if (l == m) return pmm;
-    double p1 = x * (2.0 * m + 1.0) * pmm;
-    if (l == m + 1) return p1;
-    double pl = 0.0;
-    for (int ll = m + 2; ll <= l; ++ll) {
-        pl = ((2.0 * ll - 1.0) * x * p1 - (ll + m - 1.0) * pmm) / double(ll - m);
-        pmm = p1;
-        p1 = pl;
-    }
-    return pl;
-}
+constexpr double kPi = kPiD;
 
 // One entry of the SHORE design matrix. Column col maps to (n, l, slot)
 // in the order of src/basis/shore.rs:29-44; the angular factor follows the

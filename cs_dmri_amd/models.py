@@ -144,7 +144,8 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
               alpha=1.0, alpha_ratio=1e-3, path_n_alphas=20, path_eps=None, slack=0.05, max_iter=1000, tol=1e-6,
               non_negative=False, lambda_n=1e-8, lambda_l=1e-8, nonneg_max_iter=200, nonneg_tol=1e-9,
               nonneg_epsilon=1e-10, diagnostics=False, allow_single_shell=False, n_threads=None,
-              precision="fp32", refresh_every=10) -> dict:
+              precision="fp32", refresh_every=10, graddev=None, bvecs_vox=None, frame_rotation=None,
+              graddev_variant=-1) -> dict:
     """Drop-in for ``_cs_dmri.shore_fit`` (python/src/models.rs:70-198).
 
     Same keywords, defaults, validation messages (``ValueError``) and returned
@@ -154,6 +155,13 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
     ``bic``, ``rss_l2`` where the solver has them). ``nonneg_*`` and
     ``n_threads`` are accepted for signature parity; ``precision`` and
     ``refresh_every`` select the device arithmetic.
+
+    ``graddev`` (X,Y,Z,3,3) or (X,Y,Z,9), a row-major gradient-deviation
+    tensor T per voxel with g_eff = T^T g (src/graddev.rs:1-24), fits every
+    voxel whose T is not the identity with its own effective table and design
+    (src/graddev.rs:160-185). ``bvecs_vox`` are the voxel-axis directions
+    (default ``bvecs``) and ``frame_rotation`` R maps them to ``bvecs``
+    (default identity). L1 with ``alpha_mode`` fixed or alpha-ratio only.
     """
     if bvec_frame not in ("image-axis", "world-ras"):
         raise ValueError(f"bvec_frame must be 'image-axis' or 'world-ras', got {bvec_frame!r}")
@@ -191,7 +199,13 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
                       ("alpha", amap), ("bic", bic), ("rss_l2", rss_l2)):
         setattr(out, name, None if arr is None else arr.ctypes.data)
     v = L.ShorefitVolume(vol.ctypes.data, m.ctypes.data, nx, ny, nz, 0, 0)
-    L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), None))
+    if graddev is None:
+        L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), None))
+    else:
+        gd, _keep_gd = _graddev_struct(graddev, (nx, ny, nz), g.n_grads, bvecs_vox, frame_rotation,
+                                       graddev_variant)
+        L.check(L.lib().shorefit_fit_graddev(C.byref(g), C.byref(v), C.byref(gd), C.byref(spec), C.byref(out),
+                                             None))
     dist = tuple(out.alpha_distribution) if out.has_alpha_distribution else None
     result = {
         "coefficients": coefs,
@@ -214,6 +228,81 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
         if rss_l2 is not None:
             result["rss_l2"] = rss_l2
     return result
+
+
+def _graddev_struct(graddev, shape, n_grads, bvecs_vox=None, frame_rotation=None, variant=-1):
+    f = np.asarray(graddev)
+    if f.shape[:3] != tuple(shape) or f.shape[3:] not in ((3, 3), (9,)):
+        raise ValueError(f"graddev must have shape {tuple(shape)} + (3, 3) or (9,), got {f.shape}")
+    f = np.ascontiguousarray(f.reshape(tuple(shape) + (9,)), dtype=np.float32)
+    gv = None
+    if bvecs_vox is not None:
+        gv = np.ascontiguousarray(bvecs_vox, dtype=np.float64)
+        if gv.shape != (n_grads, 3):
+            raise ValueError(f"bvecs_vox must have shape ({n_grads}, 3), got {gv.shape}")
+    R = np.eye(3) if frame_rotation is None else np.asarray(frame_rotation, dtype=np.float64)
+    if R.shape != (3, 3):
+        raise ValueError(f"frame_rotation must be 3x3, got {R.shape}")
+    gd = L.ShorefitGradDev()
+    gd.field = f.ctypes.data
+    gd.bvecs_vox = L.ptr(gv, L._pd)
+    gd.frame_rotation[:] = [float(x) for x in R.ravel()]
+    gd.variant = int(variant)
+    return gd, (f, gv)
+
+
+def graddev_variants(radial_order) -> list[dict]:
+    """Tile-shape variants of the gradient-deviation kernels at radial_order."""
+    out = []
+    n = L.lib().shorefit_graddev_variants(int(radial_order), -1, None, None, None, None)
+    for i in range(n):
+        v = [C.c_int32() for _ in range(4)]
+        L.lib().shorefit_graddev_variants(int(radial_order), i, *[C.byref(x) for x in v])
+        out.append({"kp": v[0].value, "rows": v[1].value, "gram_waves": v[2].value, "fista_waves": v[3].value})
+    return out
+
+
+def graddev_tune(data, bvals, bvecs, mask, graddev, *, big_delta=None, small_delta=None, b0_threshold=50.0,
+                 radial_order=6, zeta=700.0, alpha_mode="fixed", alpha=1e-3, alpha_ratio=1e-3, max_iter=1000,
+                 tol=1e-6, bvecs_vox=None, frame_rotation=None, max_voxels=16384) -> dict:
+    """Autotune the gradient-deviation kernels on (a sample of) this volume:
+    returns {"variant": chosen index, "ms": per-variant device time}; later
+    fits with the same radial order and gradient count use the choice."""
+    vol = np.ascontiguousarray(data, dtype=np.float32)
+    g, _keep = L.gradients(bvals, bvecs, big_delta, small_delta, b0_threshold)
+    m = np.ascontiguousarray(np.asarray(mask).astype(np.uint8))
+    nx, ny, nz, _ = vol.shape
+    spec = _spec(radial_order, zeta, "l1", alpha_mode, alpha, alpha_ratio, 20, None, 0.05, max_iter, tol, False,
+                 1e-8, 1e-8, False, False, "fp32", 10)
+    gd, _keep_gd = _graddev_struct(graddev, (nx, ny, nz), g.n_grads, bvecs_vox, frame_rotation)
+    n = L.lib().shorefit_graddev_variants(int(radial_order), -1, None, None, None, None)
+    ms = np.zeros(max(n, 1))
+    v = L.ShorefitVolume(vol.ctypes.data, m.ctypes.data, nx, ny, nz, 0, 0)
+    rc = L.lib().shorefit_graddev_tune(C.byref(g), C.byref(v), C.byref(gd), C.byref(spec), int(max_voxels),
+                                       L.ptr(ms, L._pd), None)
+    if rc < 0:
+        L.check(-rc)
+    return {"variant": rc, "ms": ms[:n].tolist()}
+
+
+def graddev_gram(data, bvals, bvecs, graddev, voxels, *, big_delta=None, small_delta=None, b0_threshold=50.0,
+                 radial_order=6, zeta=700.0, bvecs_vox=None, frame_rotation=None):
+    """Per-voxel G = A_v^T A_v (float32), b = A_v^T s and Lipschitz constants
+    of the gradient-deviation design for linear voxel indices ``voxels``."""
+    vol = np.ascontiguousarray(data, dtype=np.float32)
+    g, _keep = L.gradients(bvals, bvecs, big_delta, small_delta, b0_threshold)
+    nx, ny, nz, _ = vol.shape
+    gd, _keep_gd = _graddev_struct(graddev, (nx, ny, nz), g.n_grads, bvecs_vox, frame_rotation)
+    vox = np.ascontiguousarray(voxels, dtype=np.int64).ravel()
+    K = n_coeffs(radial_order)
+    G = np.zeros((len(vox), K, K), np.float32)
+    b = np.zeros((len(vox), K))
+    lip = np.zeros(len(vox))
+    v = L.ShorefitVolume(vol.ctypes.data, None, nx, ny, nz, 0, 0)
+    L.check(L.lib().shorefit_graddev_gram(C.byref(g), C.byref(v), C.byref(gd), int(radial_order), float(zeta),
+                                          vox.ctypes.data_as(C.POINTER(C.c_int64)), len(vox), L.ptr(G, L._pf),
+                                          L.ptr(b, L._pd), L.ptr(lip, L._pd)))
+    return G, b, lip
 
 
 def _coef_volume(coefficients, radial_order):

@@ -161,3 +161,19 @@ def graddev_tensors(n_vox: int, seed: int = 4, sd: float = 0.02) -> np.ndarray:
     E[:, iu[0], iu[1]] = e6
     E = E + np.transpose(E, (0, 2, 1)) - np.eye(3)[None] * E
     return np.eye(3)[None] + E
+
+
+def config4(shape=(96, 96, 60), radial_order: int = 8):
+    """Uneven per-voxel-design workload: config 2's table and ellipsoid mask,
+    per-voxel SNR ~ U(5, 40) (seed 5) and deviation tensors T = I + E (seed
+    4) on the masked voxels (identity outside). The signals follow the
+    nominal table. Returns (workload, field (X, Y, Z, 3, 3) float32)."""
+    gt = cartesian_table(257, seed=3)
+    mask = ellipsoid_mask(shape) if tuple(shape) == (96, 96, 60) else np.ones(shape, dtype=bool)
+    n = int(mask.sum())
+    sig = mixed_snr_signals(gt, n, seed=5)
+    field = np.broadcast_to(np.eye(3, dtype=np.float32), tuple(shape) + (3, 3)).copy()
+    field[mask] = graddev_tensors(n, seed=4).astype(np.float32)
+    wl = Workload(f"shore{radial_order}-graddev-cartesian258-{'x'.join(map(str, shape))}", gt,
+                  _volume(shape, mask, sig), mask, radial_order)
+    return wl, field

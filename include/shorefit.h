@@ -136,6 +136,60 @@ void shorefit_spec_default(ShorefitSpec* spec);
 int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitSpec* spec,
                  ShorefitOutput* out, void* stream);
 
+/* ------------------------------------------------ gradient deviation ----
+ * A gradient-deviation ("graddev") field gives, per voxel, a row-major 3x3 T
+ * with g_eff = T^T g in the field's voxel axes (src/graddev.rs:1-24). Each
+ * voxel whose T differs from the identity by IDENTITY_TOL = 1e-6 or more
+ * (src/graddev.rs:35-36, :89-93) is fitted with its own effective table
+ * (src/graddev.rs:160-185) and design matrix, built on the device and never
+ * stored; the others use the nominal design, exactly as without a field. The
+ * reference applies the field to its DTI / CSD fits through this per-voxel
+ * pattern (src/dti.rs:393-407); its SHORE fit takes no field, so this entry
+ * point extends fit_shore (src/fit.rs:687-805) with one. */
+typedef struct ShorefitGradDev {
+    const float* field;         /* (X,Y,Z,9) float32 row-major T, identity included;
+                                   device memory when vol->on_device */
+    const double* bvecs_vox;    /* N x 3 voxel-axis bvecs (DwiData::bvecs_vox, src/io/dwi.rs:33),
+                                   host memory; NULL = grads->bvecs */
+    double frame_rotation[9];   /* row-major R, grads->bvecs[i] = R bvecs_vox[i]
+                                   (DwiData::frame_rotation, src/io/dwi.rs:34-37; identity for image-axis) */
+    int32_t variant;            /* tile-shape variant for this radial order, or -1: the tuned one
+                                   (shorefit_graddev_tune) else the default */
+    int32_t reserved;
+} ShorefitGradDev;
+
+/* fit_shore with a gradient-deviation field. L1 with alpha_mode fixed or
+ * alpha-ratio, radial order 2..10 (else SHOREFIT_E_UNSUPPORTED). Outputs as
+ * shorefit_fit; out->lipschitz is the nominal design's. */
+int shorefit_fit_graddev(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
+                         const ShorefitSpec* spec, ShorefitOutput* out, void* stream);
+
+/* Number of tile-shape variants of the gradient-deviation kernels at
+ * radial_order, and (when the pointers are not NULL) variant i's padded
+ * coefficient count, Gram rows per chunk, Gram waves and FISTA waves. */
+int shorefit_graddev_variants(uint32_t radial_order, int32_t i, int32_t* kp, int32_t* rows, int32_t* gram_waves,
+                              int32_t* fista_waves);
+
+/* Autotuner: fits the first min(max_voxels, deviated voxels) voxels with
+ * every variant of spec->radial_order, stores the fastest for (radial order,
+ * N) in this process (used by shorefit_fit_graddev with variant -1) and
+ * writes each variant's device time in ms to ms_out (n_variants entries, may
+ * be NULL). Returns the chosen variant's index, or a negative status. */
+int shorefit_graddev_tune(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
+                          const ShorefitSpec* spec, int64_t max_voxels, double* ms_out, void* stream);
+
+/* Per-voxel set-up of the gradient-deviation fit (FistaSolver::new and the
+ * alpha_max right-hand side for A_v, src/solver/fista.rs:38-62,
+ * src/solver/alpha.rs:47-53): for each listed voxel (linear index into the
+ * field and the volume, which must be host memory), G = A_v^T A_v (K x K,
+ * float32 of an FP64 accumulation), b = A_v^T s (K, float64) and the
+ * Lipschitz constant L = max(sigma_max(A_v)^2 / N, eps) the device solver
+ * uses. Host outputs; any may be NULL. The field is used as given (no
+ * identity test). */
+int shorefit_graddev_gram(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
+                          uint32_t radial_order, double zeta, const int64_t* voxels, int64_t n_voxels,
+                          float* gram_out, double* rhs_out, double* lipschitz_out);
+
 /* Number of SHORE coefficients at radial_order (ShoreBasis::n_coeffs,
  * src/basis/shore.rs:29-44, :118-120): 72 / 145 / 256 at order 6 / 8 / 10.
  * Returns -1 for an odd order. */

@@ -73,6 +73,9 @@ class Oracle:
                                     C.c_uint32, _d, _i, _pd, _pd, _d, _d, _i, _pf, _pf, _pf, _pu32, _pu8,
                                     _pf, _pf, _pf, _pd]),
             "orc_effective_table": (None, [_pd, _pd, _pd, _pd, _pd, _i, _d, _pd, _pd]),
+            "orc_fit_graddev": (_i, [_pf, _pu8, _i, _i, _i, _i, _pd, _pd, _pd, _pd, _d, _d, _d, _u, _d, _pf,
+                                     _i, _d, _d, C.c_uint32, _d, _i, _i, _pf, _pf, _pf, _pu32, _pu8, _pf, _pf,
+                                     _pd]),
             "orc_hyp2f1": (_d, [_d, _d, _d, _d]),
             "orc_binomial_float": (_d, [_d, _u]),
             "orc_odf_radial_factor": (_d, [_u, _u, _d]),
@@ -253,6 +256,43 @@ class Oracle:
         self.lib.orc_effective_table(_ptr(T, _pd), _ptr(R, _pd), _ptr(b, _pd), _ptr(gv, _pd), _ptr(gn, _pd),
                                      len(b), b0_threshold, _ptr(bo, _pd), _ptr(do, _pd))
         return bo, do
+
+    def fit_graddev(self, data, mask, field, bvals, bvecs_vox, *, bvecs_nominal=None, frame_rotation=None,
+                    big_delta, small_delta, b0_threshold=50.0, radial_order=6, zeta=700.0, alpha_mode="fixed",
+                    alpha=1e-3, alpha_ratio=1e-3, max_iter=500, tol=1e-6, non_negative=False,
+                    n_threads=0) -> dict:
+        """Per-voxel-design L1 fit (src/graddev.rs + src/dti.rs:393-407
+        pattern): field (X,Y,Z,3,3) or (X,Y,Z,9) row-major T."""
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        nx, ny, nz, N = data.shape
+        fld = np.ascontiguousarray(field, dtype=np.float32).reshape(nx, ny, nz, 9)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        R = _f64(np.eye(3) if frame_rotation is None else frame_rotation).reshape(9)
+        gv = _f64(bvecs_vox).reshape(-1, 3)
+        gn = _f64(gv @ R.reshape(3, 3).T if bvecs_nominal is None else bvecs_nominal).reshape(-1, 3)
+        b = _f64(bvals)
+        K = self.n_coeffs(radial_order)
+        shp = (nx, ny, nz)
+        out = {
+            "coefficients": np.zeros(shp + (K,), np.float32),
+            "r2": np.zeros(shp, np.float32),
+            "residual_l2": np.zeros(shp, np.float32),
+            "iterations": np.zeros(shp, np.uint32),
+            "regularization_kind": np.zeros(shp, np.uint8),
+            "alpha": np.zeros(shp, np.float32),
+            "bic": np.zeros(shp, np.float32),
+            "lipschitz": np.zeros(shp, np.float64),
+        }
+        rc = self.lib.orc_fit_graddev(
+            _ptr(data, _pf), _ptr(m, _pu8), nx, ny, nz, N, _ptr(b, _pd), _ptr(gv, _pd), _ptr(gn, _pd),
+            _ptr(R, _pd), big_delta, small_delta, b0_threshold, radial_order, zeta, _ptr(fld, _pf),
+            STRATEGIES[alpha_mode], alpha, alpha_ratio, max_iter, tol, int(non_negative), int(n_threads),
+            _ptr(out["coefficients"], _pf), _ptr(out["r2"], _pf), _ptr(out["residual_l2"], _pf),
+            _ptr(out["iterations"], _pu32), _ptr(out["regularization_kind"], _pu8), _ptr(out["alpha"], _pf),
+            _ptr(out["bic"], _pf), _ptr(out["lipschitz"], _pd))
+        if rc != 0:
+            raise ValueError("graddev oracle supports alpha_mode fixed and alpha-ratio only")
+        return out
 
     # -- post-fit (src/odf.rs, src/scalars/microstructure.rs, src/synth.rs) --
     def hyp2f1(self, a, b, c, z) -> float:

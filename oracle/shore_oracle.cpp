@@ -1055,6 +1055,101 @@ void orc_effective_table(const double* T, const double* R, const double* bvals, 
     }
 }
 
+// Whole-volume L1 fit with a gradient-deviation field: the per-voxel loop of
+// src/dti.rs:393-407 (the reference's pattern for graddev fits) applied to
+// the SHORE fit. A voxel whose T is the identity to IDENTITY_TOL
+// (src/graddev.rs:35-36, :89-93) is fitted with the nominal design; any other
+// voxel gets the effective table (src/graddev.rs:160-185), q-values from that
+// table (src/qspace.rs:130-149), its own design (src/basis/shore.rs:122-178),
+// its own Lipschitz constant (src/solver/fista.rs:38-62) and then the fixed /
+// alpha-ratio resolve (src/solver/alpha.rs:207-233), as the volume driver
+// does (src/fit.rs:219-362). field: (X,Y,Z,9) f32 row-major T. strategy:
+// 0 fixed, 1 alpha-ratio. lip_out (optional, f64 per voxel) receives the
+// Lipschitz constant each fitted voxel used.
+int orc_fit_graddev(const float* data, const uint8_t* mask, int nx, int ny, int nz, int N, const double* bvals,
+                    const double* bvecs_vox, const double* bvecs_nominal, const double* R, double big_delta,
+                    double small_delta, double b0_thr, unsigned radial_order, double zeta, const float* field,
+                    int strategy, double alpha, double alpha_ratio, uint32_t max_iter, double tol, int non_negative,
+                    int n_threads, float* coef_out, float* r2_out, float* resid_out, uint32_t* iters_out,
+                    uint8_t* kind_out, float* alpha_out, float* bic_out, double* lip_out) {
+    if (strategy != kFixed && strategy != kAlphaRatio) return -2;
+    const int K = int(shore_indices(radial_order).size());
+    std::vector<int64_t> work;
+    for (int64_t x = 0; x < nx; ++x)
+        for (int64_t y = 0; y < ny; ++y)
+            for (int64_t z = 0; z < nz; ++z) {
+                const int64_t vi = (x * ny + y) * nz + z;
+                if (!mask || mask[vi]) work.push_back(vi);
+            }
+    const int64_t nvox = int64_t(nx) * ny * nz;
+    std::fill(coef_out, coef_out + nvox * K, 0.0f);
+    if (r2_out) std::fill(r2_out, r2_out + nvox, 0.0f);
+    if (resid_out) std::fill(resid_out, resid_out + nvox, 0.0f);
+    if (iters_out) std::fill(iters_out, iters_out + nvox, 0u);
+    if (kind_out) std::fill(kind_out, kind_out + nvox, uint8_t(0));
+    if (alpha_out) std::fill(alpha_out, alpha_out + nvox, 0.0f);
+    if (bic_out) std::fill(bic_out, bic_out + nvox, 0.0f);
+    if (lip_out) std::fill(lip_out, lip_out + nvox, 0.0);
+
+    std::vector<double> q0(N), A0(size_t(N) * K);
+    qvals(bvals, N, big_delta, small_delta, b0_thr, q0.data());
+    design_matrix(radial_order, zeta, q0.data(), bvecs_nominal, N, A0.data());
+    const double L0 = lipschitz(A0.data(), N, K);
+
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        std::vector<double> s(N), c(K), beff(N), deff(size_t(N) * 3), q(N), Av(size_t(N) * K);
+        for (;;) {
+            const size_t w = next.fetch_add(1);
+            if (w >= work.size()) break;
+            const int64_t vi = work[w];
+            double T[9];
+            double dev = 0.0;
+            for (int k = 0; k < 9; ++k) {
+                T[k] = double(field[vi * 9 + k]);
+                dev = std::max(dev, std::fabs(T[k] - ((k % 4 == 0) ? 1.0 : 0.0)));
+            }
+            const double* A = A0.data();
+            double L = L0;
+            if (!(dev < 1e-6)) {
+                orc_effective_table(T, R, bvals, bvecs_vox, bvecs_nominal, N, b0_thr, beff.data(), deff.data());
+                qvals(beff.data(), N, big_delta, small_delta, b0_thr, q.data());
+                design_matrix(radial_order, zeta, q.data(), deff.data(), N, Av.data());
+                A = Av.data();
+                L = lipschitz(A, N, K);
+            }
+            const float* src = data + vi * N;
+            double mean = 0.0;
+            for (int i = 0; i < N; ++i) {
+                s[i] = double(src[i]);
+                mean += s[i];
+            }
+            mean /= double(N);
+            Problem P{A, N, K, L, max_iter, tol, non_negative != 0, strategy, alpha, alpha_ratio, 0.0, 0.0, 0,
+                      nullptr};
+            const VoxelResult r = resolve(P, s.data(), c.data());
+            double ss_tot = 0.0;
+            for (int i = 0; i < N; ++i) ss_tot += (s[i] - mean) * (s[i] - mean);
+            const double ss_res = r.residual_l2 * r.residual_l2;
+            const double r2 = ss_tot > 0.0 ? 1.0 - ss_res / ss_tot : 0.0;
+            for (int k = 0; k < K; ++k) coef_out[vi * K + k] = float(c[k]);
+            if (r2_out) r2_out[vi] = float(r2);
+            if (resid_out) resid_out[vi] = float(r.residual_l2);
+            if (iters_out) iters_out[vi] = r.iterations;
+            if (kind_out) kind_out[vi] = 1;
+            if (alpha_out) alpha_out[vi] = float(r.alpha);
+            if (bic_out) bic_out[vi] = float(r.bic);
+            if (lip_out) lip_out[vi] = L;
+        }
+    };
+    if (n_threads <= 0) n_threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < n_threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    return 0;
+}
+
 // ---------------------------------------------------- post-fit entry points ----
 double orc_hyp2f1(double a, double b, double c, double z) { return hyp2f1(a, b, c, z); }
 double orc_binomial_float(double n, unsigned k) { return binomial_float(n, k); }
