@@ -64,7 +64,7 @@ class ShorefitOutput(C.Structure):
         ("big_delta", C.c_double), ("small_delta", C.c_double), ("tau", C.c_double),
         ("delta_source", C.c_int32), ("n_shells", C.c_int32), ("dwi_shells", C.c_double * 32),
         ("n_warnings", C.c_int32), ("warnings", C.c_char * 2048), ("n_voxels", C.c_int64),
-        ("lipschitz", C.c_double), ("fit_ms", C.c_double),
+        ("lipschitz", C.c_double), ("fit_ms", C.c_double), ("graddev_ms", C.c_double * 3),
     ]
 
     def warning_list(self) -> list[str]:

@@ -556,9 +556,11 @@ int64_t gd_chunk_voxels(const GdVariant& v, int64_t V) {
 }
 
 // The deviated voxels vox[0..V): Gram, FISTA and residual kernels per chunk.
-// lip (optional, device, V entries) receives each voxel's Lipschitz constant.
+// lip (optional, device, V entries) receives each voxel's Lipschitz constant;
+// phase_ms (optional) accumulates the three kernels' device times.
 void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, const float* data, int64_t sig_stride,
-            const float* field, const int64_t* vox, int64_t V, const GdRunOut& o, double* lip, hipStream_t s) {
+            const float* field, const int64_t* vox, int64_t V, const GdRunOut& o, double* lip, hipStream_t s,
+            double* phase_ms = nullptr) {
     if (V <= 0) return;
     const int N = S.tab.N, K = S.B.K, KP = var.kp;
     // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112)
@@ -572,19 +574,24 @@ void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, cons
     DevBuf<float> d_mom(mom.size());
     SF_HIP(hipMemcpyAsync(d_mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
     const int64_t Vb = gd_chunk_voxels(var, V);
-    DevBuf<float> G(size_t(Vb) * KP * KP), av(Vb);
-    DevBuf<double> b(size_t(Vb) * KP);
-    DevBuf<uint32_t> itsb(Vb);
-    DevBuf<uint8_t> st(Vb);
+    DevBuf<float> G(size_t(Vb) * KP * KP);
+    DevBuf<double> b(size_t(Vb) * KP), sstat(size_t(Vb) * 2);
+    DevBuf<GdSlotOut> so(Vb);
     DevBuf<unsigned> ctr(1);
+    hipEvent_t ev[3] = {};
+    if (phase_ms)
+        for (auto& e : ev) SF_HIP(hipEventCreate(&e));
     for (int64_t c0 = 0; c0 < V; c0 += Vb) {
         const int64_t n = std::min(Vb, V - c0);
-        GdChunk c{S.B, S.tab, data, sig_stride, field, vox + c0, n, G.p, b.p};
+        GdChunk c{S.B, S.tab, data, sig_stride, field, vox + c0, n, G.p, b.p, sstat.p};
+        if (phase_ms) SF_HIP(hipEventRecord(ev[0], s));
         gd_launch_gram(var, c, s);
+        if (phase_ms) SF_HIP(hipEventRecord(ev[1], s));
         SF_HIP(hipMemsetAsync(ctr.p, 0, sizeof(unsigned), s));
         GdFistaArgs fa{};
         fa.G = G.p;
         fa.b = b.p;
+        fa.sstat = sstat.p;
         fa.vox = vox + c0;
         fa.n = n;
         fa.N = N;
@@ -597,15 +604,23 @@ void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, cons
         fa.mom = d_mom.p;
         fa.counter = ctr.p;
         fa.coef = o.coef;
-        fa.its = itsb.p;
-        fa.status = st.p;
-        fa.alpha_v = av.p;
+        fa.slot_out = so.p;
         fa.lip = lip ? lip + c0 : nullptr;
         gd_launch_fista(var, fa, 0, s);
-        if (!o.resid && !o.r2 && !o.iters && !o.kind && !o.alpha_map && !o.bic) continue;
-        GdOutArgs oa{o.coef, itsb.p, st.p, av.p, o.resid, o.r2, o.iters, nullptr, o.kind, o.alpha_map, o.bic};
-        gd_launch_residual(c, oa, s);
+        const GdMaps maps{o.resid, o.r2, o.iters, o.kind, o.alpha_map, o.bic};
+        gd_launch_finish(so.p, sstat.p, vox + c0, n, N, maps, s);
+        if (phase_ms) {
+            SF_HIP(hipEventRecord(ev[2], s));
+            SF_HIP(hipEventSynchronize(ev[2]));
+            for (int k = 0; k < 2; ++k) {
+                float ms = 0.f;
+                SF_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                phase_ms[k] += ms;
+            }
+        }
     }
+    if (phase_ms)
+        for (auto& e : ev) (void)hipEventDestroy(e);
     SF_HIP(hipStreamSynchronize(s));  // the host momentum table goes out of scope
 }
 
@@ -615,6 +630,7 @@ void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, cons
 // voxels.
 void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitSpec* spec,
               ShorefitOutput* out, hipStream_t s, const ShorefitGradDev* gd) {
+    out->graddev_ms[0] = out->graddev_ms[1] = out->graddev_ms[2] = 0.0;
     const ShorefitSpec& sp = *spec;
     Warnings warn;
 
@@ -896,7 +912,7 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
             GdSetup S;
             gd_setup(S, *grads, big, small, *gd, sp.radial_order, sp.zeta, s);
             GdRunOut o{coef, res, r2, its, kind, amap, bic};
-            gd_run(S, *gd_var, sp, data, N, field, d_vox_dv.p, V_dv, o, nullptr, s);
+            gd_run(S, *gd_var, sp, data, N, field, d_vox_dv.p, V_dv, o, nullptr, s, out->graddev_ms);
         }
         SF_HIP(hipEventRecord(e3, s));
     } else {
@@ -1252,21 +1268,21 @@ int shorefit_graddev_gram(const ShorefitGradients* grads, const ShorefitVolume* 
         gd_setup(S, *grads, big, small, *gd, radial_order, zeta, s);
         const int KP = v.kp, K = S.B.K;
         DevBuf<float> G(size_t(n_voxels) * KP * KP);
-        DevBuf<double> b(size_t(n_voxels) * KP);
-        GdChunk c{S.B, S.tab, data, N, field, d_vox.p, n_voxels, G.p, b.p};
+        DevBuf<double> b(size_t(n_voxels) * KP), sstat(size_t(n_voxels) * 2);
+        GdChunk c{S.B, S.tab, data, N, field, d_vox.p, n_voxels, G.p, b.p, sstat.p};
         gd_launch_gram(v, c, s);
         std::vector<double> lip(n_voxels);
         if (lipschitz_out) {
             // the FISTA kernel's set-up (Lanczos) with no iterations
             DevBuf<double> d_lip(n_voxels);
-            DevBuf<float> coef(size_t(nvox) * K), av(n_voxels), mom(1);
-            DevBuf<uint32_t> its(n_voxels);
-            DevBuf<uint8_t> st(n_voxels);
+            DevBuf<float> coef(size_t(nvox) * K), mom(1);
+            DevBuf<GdSlotOut> so(n_voxels);
             DevBuf<unsigned> ctr(1);
             SF_HIP(hipMemsetAsync(ctr.p, 0, sizeof(unsigned), s));
             GdFistaArgs fa{};
             fa.G = G.p;
             fa.b = b.p;
+            fa.sstat = sstat.p;
             fa.vox = d_vox.p;
             fa.n = n_voxels;
             fa.N = int(N);
@@ -1277,9 +1293,7 @@ int shorefit_graddev_gram(const ShorefitGradients* grads, const ShorefitVolume* 
             fa.mom = mom.p;
             fa.counter = ctr.p;
             fa.coef = coef.p;
-            fa.its = its.p;
-            fa.status = st.p;
-            fa.alpha_v = av.p;
+            fa.slot_out = so.p;
             fa.lip = d_lip.p;
             gd_launch_fista(v, fa, 0, s);
             SF_HIP(hipMemcpyAsync(lip.data(), d_lip.p, sizeof(double) * n_voxels, hipMemcpyDeviceToHost, s));

@@ -245,11 +245,22 @@ struct GdChunk {
     int64_t n;
     float* G;             // [n][Kp][Kp]
     double* b;            // [n][Kp]
+    double* sstat;        // [n][2]: s's and the sum of squares about the mean
+};
+
+// Per-slot results of the FISTA kernel.
+struct GdSlotOut {
+    double rss;           // |s - A x|^2 (Gram identity, FP64)
+    float alpha;
+    uint32_t its;
+    uint16_t nnz;
+    uint16_t flags;       // bit 0 converged, bit 1 all-zero short circuit
 };
 
 struct GdFistaArgs {
     const float* G;
     const double* b;
+    const double* sstat;
     const int64_t* vox;
     int64_t n;
     int N, K;
@@ -260,22 +271,16 @@ struct GdFistaArgs {
     int nonneg;
     const float* mom;     // [max_iter] (t_k - 1) / t_{k+1}
     unsigned* counter;
-    float* coef;          // volume, [.. x K]
-    uint32_t* its;        // [n]
-    uint8_t* status;      // [n] bit 0 converged, bit 1 all-zero short circuit
-    float* alpha_v;       // [n]
+    float* coef;          // volume [.. x K], indexed by vox[slot]
+    GdSlotOut* slot_out;  // [n]
     double* lip;          // [n] optional
 };
 
-struct GdOutArgs {
-    const float* coef;
-    const uint32_t* its;
-    const uint8_t* status;
-    const float* alpha_v;
+// Output maps (indexed by voxel), any may be null.
+struct GdMaps {
     float* resid;
     float* r2;
     uint32_t* iters;
-    uint8_t* converged;
     uint8_t* kind;
     float* alpha_map;
     float* bic;
@@ -289,7 +294,8 @@ struct GdVariant {
 int gd_variants(uint32_t order, const GdVariant** first);
 void gd_launch_gram(const GdVariant& v, const GdChunk& c, hipStream_t s);
 void gd_launch_fista(const GdVariant& v, const GdFistaArgs& a, int max_blocks, hipStream_t s);
-void gd_launch_residual(const GdChunk& c, const GdOutArgs& o, hipStream_t s);
+void gd_launch_finish(const GdSlotOut* so, const double* sstat, const int64_t* vox, int64_t n, int N,
+                      const GdMaps& m, hipStream_t s);
 void gd_launch_identity(const float* field, const int64_t* vox, int64_t n, uint8_t* ident, uint8_t* devi,
                         hipStream_t s);
 void launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t n, int64_t* dst, hipStream_t s);

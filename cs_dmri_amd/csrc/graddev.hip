@@ -162,33 +162,78 @@ __device__ inline RowGeom gd_row(const GdTable& t, const double (&T)[9], int i, 
     return g;
 }
 
-__device__ inline double upow_half_l(double u, int l) {
-    double p = 1.0;
-    for (int j = 0; j < l / 2; ++j) p *= u;
-    return p;
+// Radial factors of one row for one even l: pairs (n, l), n = l..R, from the
+// Laguerre recurrence in n - l (the recurrence of laguerre_d), times
+// kappa(n, l) e^(-u/2) u^(l/2) (src/basis/shore.rs:56-70). Pair (n, l) is
+// column P_before(n) + l / 2 of the factor row, P_before(n) = sum over n' < n
+// of (n'/2 + 1).
+template <int R>
+__device__ inline void radial_task(const GdBasis& B, int l, double u, double e, double* fac) {
+    const double a = l + 0.5;
+    double ep = e;
+    for (int j = 0; j < l / 2; ++j) ep *= u;
+    double lp = 0.0, lc = 1.0;
+    int p = l + (l / 2) * (l / 2 - 1) + l / 2;
+    for (int n = l; n <= R; ++n) {
+        const int k = n - l;
+        if (k == 1) {
+            lp = 1.0;
+            lc = 1.0 + a - u;
+        } else if (k >= 2) {
+            const double nw = ((2.0 * (k - 1) + 1.0 + a - u) * lc - ((k - 1) + a) * lp) / double(k);
+            lp = lc;
+            lc = nw;
+        }
+        fac[p] = B.kappa[p] * lc * ep;
+        p += n / 2 + 1;
+    }
 }
 
-// Radial factor of pair p (src/basis/shore.rs:64-70).
-__device__ inline double gd_radial(const GdBasis& B, int p, double u, double e) {
-    const int n = B.pair_n[p], l = B.pair_l[p];
-    return B.kappa[p] * laguerre_d(n - l, l + 0.5, u) * e * upow_half_l(u, l);
-}
-
-// Angular factor a (BrainSuite order, src/basis/shore.rs:76-115); cos and
-// sin of k theta by angle addition from cos and sin of theta.
-__device__ inline double gd_angular(const GdBasis& B, int a, double cp, double ct, double st) {
-    const int kind = B.ang_kind[a];
-    if (kind == 0) return B.ang_c[a];
-    const int l = B.ang_l[a], k = B.ang_k[a];
-    const double p = legendre_d(l, k, cp);
-    if (kind == 2) return B.ang_c[a] * p;
+// Angular factors of one row for one order k = |m|: P_l^k(cos phi) for
+// l = k..R by the upward recurrence of legendre_d, times the BrainSuite
+// normalisation and cos / sin of k theta (src/basis/shore.rs:76-115), into the
+// slots l - k and l + k of every even l >= max(k, 2) (and l = 0 for k = 0).
+template <int R>
+__device__ inline void angular_task(const GdBasis& B, int k, double x, double ct, double st, double* fac) {
     double ck = 1.0, sk = 0.0;
     for (int j = 0; j < k; ++j) {
         const double cn = ck * ct - sk * st;
         sk = sk * ct + ck * st;
         ck = cn;
     }
-    return B.ang_c[a] * p * (kind == 1 ? ck : sk);
+    auto emit = [&](int l, double pv) {
+        if (l & 1) return;
+        if (l == 0) {
+            fac[0] = B.ang_c[0];
+            return;
+        }
+        const int base = l * (l - 1) / 2;
+        if (k == 0) {
+            fac[base + l] = B.ang_c[base + l] * pv;
+        } else {
+            fac[base + l - k] = B.ang_c[base + l - k] * pv * ck;
+            fac[base + l + k] = B.ang_c[base + l + k] * pv * sk;
+        }
+    };
+    double pmm = 1.0;
+    if (k > 0) {
+        const double sq = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+        double f = 1.0;
+        for (int i = 0; i < k; ++i) {
+            pmm *= -f * sq;
+            f += 2.0;
+        }
+    }
+    emit(k, pmm);
+    if (k + 1 > R) return;
+    double p1 = x * (2.0 * k + 1.0) * pmm;
+    emit(k + 1, p1);
+    for (int ll = k + 2; ll <= R; ++ll) {
+        const double pl = ((2.0 * ll - 1.0) * x * p1 - (ll + k - 1.0) * pmm) / double(ll - k);
+        pmm = p1;
+        p1 = pl;
+        emit(ll, pl);
+    }
 }
 
 __device__ inline void load_T(const float* field, int64_t v, double (&T)[9]) {
@@ -196,13 +241,43 @@ __device__ inline void load_T(const float* field, int64_t v, double (&T)[9]) {
     for (int k = 0; k < 9; ++k) T[k] = double(field[v * 9 + k]);
 }
 
+// ------------------------------------------------------------ reductions ----
+// Sum over the 16 lanes of a row (DPP): every lane ends with the same value.
+__device__ __forceinline__ float row16_sum(float x) {
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));  // quad [1,0,3,2]
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));  // quad [2,3,0,1]
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false)); // half mirror
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false)); // row mirror
+    return x;
+}
+
+// Wave sum of a value that is uniform within each row of 16 lanes.
+__device__ __forceinline__ float rows4_sum(float x) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
+}
+
+__device__ __forceinline__ double wave_sum_d(double x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x += __shfl_xor(x, m);
+    return x;
+}
+__device__ __forceinline__ double wave_max_d(double x) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x = fmax(x, __shfl_xor(x, m));
+    return x;
+}
+
 // ------------------------------------------------------------ Gram kernel ----
 template <int R, int KP, int RC, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES, 2)
     gd_gram_kernel(const GdBasis B, const GdTable tab, const float* __restrict__ data, int64_t sig_stride,
                    const float* __restrict__ field, const int64_t* __restrict__ vox, int64_t n,
-                   float* __restrict__ Gws, double* __restrict__ bws) {
+                   float* __restrict__ Gws, double* __restrict__ bws, double* __restrict__ sstat) {
     constexpr int K = n_coeffs_c(R), P = n_pairs_c(R), Q = n_ang_c(R), PQ = P + Q;
+    constexpr int TR = R / 2 + 1 + R + 1;  // factor tasks per row: radial (one per even l), angular (one per |m|)
     constexpr int KT = KP / 16, NT = KT * (KT + 1) / 2, TPW = (NT + WAVES - 1) / WAVES;
     constexpr int LDA = KP + 2;
     constexpr int NTH = 64 * WAVES;
@@ -235,6 +310,26 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
         s_st[i] = g.st;
         s_sig[i] = double(data[v * sig_stride + i]);
     }
+    __syncthreads();
+    if (w == 0) {
+        // |s|^2 and the total sum of squares about the mean (FP64), for the
+        // residual by the Gram identity and R^2 (src/fit.rs:120-128)
+        double sum = 0.0, sq = 0.0;
+        for (int i = lane; i < N; i += 64) {
+            sum += s_sig[i];
+            sq += s_sig[i] * s_sig[i];
+        }
+        sum = wave_sum_d(sum);
+        sq = wave_sum_d(sq);
+        const double mean = sum / double(N);
+        double sst = 0.0;
+        for (int i = lane; i < N; i += 64) sst += (s_sig[i] - mean) * (s_sig[i] - mean);
+        sst = wave_sum_d(sst);
+        if (lane == 0) {
+            sstat[2 * slot] = sq;
+            sstat[2 * slot + 1] = sst;
+        }
+    }
 
     // tiles of this wave: t = w + WAVES j in row-major order of the upper
     // triangle (I <= J)
@@ -255,28 +350,30 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
     double bacc = 0.0;
 
     for (int i0 = 0; i0 < N; i0 += RC) {
-        __syncthreads();  // geometry ready; previous chunk's A reads done
-        for (int it = tid; it < RC * PQ; it += NTH) {
-            const int r = it / PQ, f = it - r * PQ;
+        __syncthreads();  // previous chunk's A reads done
+        const int rmax = min(RC, N - i0);
+        for (int t = tid; t < rmax * TR; t += NTH) {
+            const int r = t / TR, q = t - r * TR;
             const int i = i0 + r;
-            double val = 0.0;
-            if (i < N) val = f < P ? gd_radial(B, f, s_u[i], s_e[i]) : gd_angular(B, f - P, s_cp[i], s_ct[i], s_st[i]);
-            s_fac[it] = val;
+            if (q <= R / 2)
+                radial_task<R>(B, 2 * q, s_u[i], s_e[i], s_fac + r * PQ);
+            else
+                angular_task<R>(B, q - (R / 2 + 1), s_cp[i], s_ct[i], s_st[i], s_fac + r * PQ + P);
         }
         __syncthreads();
         for (int it = tid; it < RC * KP; it += NTH) {
             const int r = it / KP, c = it - r * KP;
             double val = 0.0;
-            if (c < K) val = s_fac[r * PQ + B.col_pair[c]] * s_fac[r * PQ + P + B.col_ang[c]];
+            if (c < K && r < rmax) val = s_fac[r * PQ + B.col_pair[c]] * s_fac[r * PQ + P + B.col_ang[c]];
             s_A[r * LDA + c] = val;
         }
         __syncthreads();
-        if (tid < KP) {
-            const int rmax = min(RC, N - i0);
+        if (tid < KP)
             for (int r = 0; r < rmax; ++r) bacc += s_A[r * LDA + tid] * s_sig[i0 + r];
-        }
+        const int ksteps = (rmax + 3) / 4;
 #pragma unroll
         for (int kk = 0; kk < RC / 4; ++kk) {
+            if (kk >= ksteps) break;
             const double* rowp = s_A + (4 * kk + (lane >> 4)) * LDA + (lane & 15);
 #pragma unroll
             for (int j = 0; j < TPW; ++j) {
@@ -309,40 +406,12 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
 
 // ------------------------------------------------------------ FISTA kernel ----
 
-// Sum over the 16 lanes of a row (DPP): every lane ends with the same value.
-__device__ __forceinline__ float row16_sum(float x) {
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));  // quad [1,0,3,2]
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x4E, 0xF, 0xF, false));  // quad [2,3,0,1]
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x141, 0xF, 0xF, false)); // half mirror
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x140, 0xF, 0xF, false)); // row mirror
-    return x;
-}
-
-// Wave sum of a value that is uniform within each row of 16 lanes.
-__device__ __forceinline__ float rows4_sum(float x) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0)) +
-           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16)) +
-           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32)) +
-           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
-}
-
-__device__ __forceinline__ double wave_sum_d(double x) {
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) x += __shfl_xor(x, m);
-    return x;
-}
-__device__ __forceinline__ double wave_max_d(double x) {
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) x = fmax(x, __shfl_xor(x, m));
-    return x;
-}
-
 template <int KP, int W>
-__global__ void __launch_bounds__(64 * W) gd_fista_kernel(const GdFistaArgs a) {
+__global__ void __launch_bounds__(64 * W, (KP >= 256 ? 2 : 3)) gd_fista_kernel(const GdFistaArgs a) {
     constexpr int RB = KP / (4 * W), CB = KP / 16;
     static_assert(RB * 4 * W == KP && CB * 16 == KP && RB <= 16, "block shape");
     __shared__ float s_vec[2][KP];
-    __shared__ double s_red[2][W][2];
+    __shared__ double s_red[2][W][3];
     __shared__ double s_al[kLanczos], s_be[kLanczos];
     __shared__ int64_t s_slot;
 
@@ -354,7 +423,7 @@ __global__ void __launch_bounds__(64 * W) gd_fista_kernel(const GdFistaArgs a) {
     const int N = a.N, K = a.K;
     int rp = 0;             // reduction buffer parity
 
-    // Block sums / max of FP64 values (one barrier each).
+    // Block sums of FP64 values (one barrier each).
     auto block_sum2 = [&](double x, double y, double& sx, double& sy) {
         x = wave_sum_d(x);
         y = wave_sum_d(y);
@@ -369,6 +438,27 @@ __global__ void __launch_bounds__(64 * W) gd_fista_kernel(const GdFistaArgs a) {
         for (int q = 0; q < W; ++q) {
             sx += s_red[rp][q][0];
             sy += s_red[rp][q][1];
+        }
+        rp ^= 1;
+    };
+    auto block_sum3 = [&](double x, double y, double z, double& sx, double& sy, double& sz) {
+        x = wave_sum_d(x);
+        y = wave_sum_d(y);
+        z = wave_sum_d(z);
+        if (lane == 0) {
+            s_red[rp][w][0] = x;
+            s_red[rp][w][1] = y;
+            s_red[rp][w][2] = z;
+        }
+        __syncthreads();
+        sx = 0.0;
+        sy = 0.0;
+        sz = 0.0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            sx += s_red[rp][q][0];
+            sy += s_red[rp][q][1];
+            sz += s_red[rp][q][2];
         }
         rp ^= 1;
     };
@@ -495,12 +585,12 @@ __global__ void __launch_bounds__(64 * W) gd_fista_kernel(const GdFistaArgs a) {
         float x = 0.f, y = 0.f;
         uint32_t its = 0;
         bool conv = false;
+        int buf = 0;
         if (zero_voxel) {
             conv = true;
         } else {
             if (owner) s_vec[0][j] = 0.f;
             __syncthreads();
-            int buf = 0;
             for (uint32_t it = 0; it < a.max_iter; ++it) {
                 const float gy = matvec(g, buf);
                 float d2 = 0.f, n2 = 0.f;
@@ -532,127 +622,46 @@ __global__ void __launch_bounds__(64 * W) gd_fista_kernel(const GdFistaArgs a) {
                 rp ^= 1;
                 buf ^= 1;
                 its = it + 1;
-                // src/solver/fista.rs:113-127: |dx| / max(|x+|, 1e-12) < tol
-                if (sqrt(D2) / fmax(sqrt(N2), 1e-12) < a.tol) {
+                // src/solver/fista.rs:113-127: |dx| / max(|x+|, 1e-12) < tol,
+                // compared in squares
+                if (D2 < a.tol * a.tol * fmax(N2, 1e-24)) {
                     conv = true;
                     break;
                 }
             }
         }
-        if (owner && j < K) a.coef[a.vox[slot] * K + j] = x;
+        const int64_t out = a.vox[slot];
+        if (owner && j < K) a.coef[out * K + j] = x;
+        // residual by the Gram identity in FP64 with the FP32 coefficients:
+        // |s - A x|^2 = s's - 2 x'(A^T s) + x'(A^T A)x
+        if (owner) s_vec[buf ^ 1][j] = x;
+        __syncthreads();
+        double xgx = 0.0;
+        {
+            float xc[CB];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) xc[c] = s_vec[buf ^ 1][c0 + c];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                float acc = 0.f;
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc = fmaf(g[r][c], xc[c], acc);
+                xgx = fma(double(s_vec[buf ^ 1][r0 + r]), double(acc), xgx);
+            }
+        }
+        double XGX, XB, NZ;
+        block_sum3(xgx, owner ? double(x) * bj : 0.0, (owner && j < K && x != 0.f) ? 1.0 : 0.0, XGX, XB, NZ);
         if (tid == 0) {
-            a.its[slot] = its;
-            a.status[slot] = uint8_t((conv ? 1 : 0) | (zero_voxel ? 2 : 0));
-            a.alpha_v[slot] = float(alpha_v);
+            GdSlotOut so;
+            so.rss = zero_voxel ? 0.0 : fmax(a.sstat[2 * slot] - 2.0 * XB + XGX, 0.0);
+            so.alpha = float(alpha_v);
+            so.its = its;
+            so.nnz = uint16_t(NZ);
+            so.flags = uint16_t((conv ? 1 : 0) | (zero_voxel ? 2 : 0));
+            a.slot_out[slot] = so;
             if (a.lip) a.lip[slot] = L;
         }
         __syncthreads();  // s_slot / s_vec reuse
-    }
-}
-
-// --------------------------------------------------------- residual kernel ----
-// One workgroup (4 waves) per voxel: the design rows again, RC at a time, as
-// radial x angular factor tables in LDS (the Gram kernel's items); wave w
-// forms rows w, w+4, ... of A_v x with the lanes over the columns.
-template <int R, int RC>
-__global__ void __launch_bounds__(256)
-    gd_residual_kernel(const GdBasis B, const GdTable tab, const float* __restrict__ data, int64_t sig_stride,
-                       const float* __restrict__ field, const int64_t* __restrict__ vox, int64_t n,
-                       const GdOutArgs o) {
-    constexpr int K = n_coeffs_c(R), P = n_pairs_c(R), Q = n_ang_c(R), PQ = P + Q;
-    const int64_t slot = blockIdx.x;
-    if (slot >= n) return;
-    const int64_t v = vox[slot];
-    const int N = tab.N;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    extern __shared__ double smem[];
-    double* s_u = smem;
-    double* s_e = s_u + N;
-    double* s_cp = s_e + N;
-    double* s_ct = s_cp + N;
-    double* s_st = s_ct + N;
-    double* s_sig = s_st + N;
-    double* s_fac = s_sig + N;  // RC x PQ
-    __shared__ double s_x[K];
-    __shared__ double s_red[4][3];
-
-    int nnz = 0;
-    for (int c = tid; c < K; c += 256) {
-        const float xc = o.coef[v * K + c];
-        s_x[c] = double(xc);
-        nnz += xc != 0.f;
-    }
-    double T[9];
-    load_T(field, v, T);
-    double ssum = 0.0;
-    for (int i = tid; i < N; i += 256) {
-        const RowGeom g = gd_row(tab, T, i, B.zeta);
-        s_u[i] = g.u;
-        s_e[i] = g.e;
-        s_cp[i] = g.cp;
-        s_ct[i] = g.ct;
-        s_st[i] = g.st;
-        const double sv = double(data[v * sig_stride + i]);
-        s_sig[i] = sv;
-        ssum += sv;
-    }
-    ssum = wave_sum_d(ssum);
-    const double nz = wave_sum_d(double(nnz));
-    if (lane == 0) {
-        s_red[w][0] = ssum;
-        s_red[w][1] = nz;
-    }
-    __syncthreads();
-    const double mean = (s_red[0][0] + s_red[1][0] + s_red[2][0] + s_red[3][0]) / double(N);
-    const int nnz_all = int(s_red[0][1] + s_red[1][1] + s_red[2][1] + s_red[3][1]);
-
-    double rss = 0.0, sst = 0.0;  // lane 0 of each wave
-    for (int i0 = 0; i0 < N; i0 += RC) {
-        __syncthreads();
-        for (int it = tid; it < RC * PQ; it += 256) {
-            const int r = it / PQ, f = it - r * PQ;
-            const int i = i0 + r;
-            double val = 0.0;
-            if (i < N) val = f < P ? gd_radial(B, f, s_u[i], s_e[i]) : gd_angular(B, f - P, s_cp[i], s_ct[i], s_st[i]);
-            s_fac[it] = val;
-        }
-        __syncthreads();
-        const int rmax = min(RC, N - i0);
-        for (int r = w; r < rmax; r += 4) {
-            double acc = 0.0;
-            for (int c = lane; c < K; c += 64)
-                acc += s_fac[r * PQ + B.col_pair[c]] * s_fac[r * PQ + P + B.col_ang[c]] * s_x[c];
-            acc = wave_sum_d(acc);
-            const double sv = s_sig[i0 + r];
-            const double e = sv - acc;
-            rss += e * e;
-            sst += (sv - mean) * (sv - mean);
-        }
-    }
-    if (lane == 0) {
-        s_red[w][0] = rss;
-        s_red[w][1] = sst;
-    }
-    __syncthreads();
-    if (tid != 0) return;
-    rss = s_red[0][0] + s_red[1][0] + s_red[2][0] + s_red[3][0];
-    sst = s_red[0][1] + s_red[1][1] + s_red[2][1] + s_red[3][1];
-    const uint8_t st = o.status[slot];
-    const bool zero_voxel = (st & 2) != 0;
-    // src/fit.rs:120-128 (R^2), src/solver/alpha.rs:99-103 (BIC), :178-189
-    const double ss_res = zero_voxel ? 0.0 : rss;
-    const double resid_l2 = sqrt(ss_res);
-    const double r2 = sst > 0.0 ? 1.0 - ss_res / sst : 0.0;
-    if (o.resid) o.resid[v] = float(resid_l2);
-    if (o.r2) o.r2[v] = float(r2);
-    if (o.iters) o.iters[v] = o.its[slot];
-    if (o.converged) o.converged[v] = st & 1;
-    if (o.kind) o.kind[v] = 1;
-    if (o.alpha_map) o.alpha_map[v] = o.alpha_v[slot];
-    if (o.bic) {
-        const double nn = double(N);
-        const double rc = fmax(resid_l2 * resid_l2, 2.2250738585072014e-308);
-        o.bic[v] = zero_voxel ? 0.f : float(nn * log(rc / nn) + log(nn) * double(nnz_all));
     }
 }
 
@@ -668,6 +677,30 @@ __global__ void gd_identity_kernel(const float* __restrict__ field, const int64_
     const bool id = dev < 1e-6;
     ident[i] = id;
     devi[i] = !id;
+}
+
+// Maps of the deviated voxels from the FISTA kernel's per-slot results
+// (src/fit.rs:120-128 R^2, src/solver/alpha.rs:99-103 BIC, :178-189 the
+// all-zero voxel), one thread per slot.
+__global__ void gd_finish_kernel(const GdSlotOut* __restrict__ so, const double* __restrict__ sstat,
+                                 const int64_t* __restrict__ vox, int64_t n, int N, GdMaps m) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GdSlotOut o = so[i];
+    const int64_t v = vox[i];
+    const bool zero_voxel = (o.flags & 2) != 0;
+    const double sst = sstat[2 * i + 1];
+    const double resid_l2 = sqrt(o.rss);
+    if (m.resid) m.resid[v] = float(resid_l2);
+    if (m.r2) m.r2[v] = float(sst > 0.0 ? 1.0 - o.rss / sst : 0.0);
+    if (m.iters) m.iters[v] = o.its;
+    if (m.kind) m.kind[v] = 1;
+    if (m.alpha_map) m.alpha_map[v] = o.alpha;
+    if (m.bic) {
+        const double nn = double(N);
+        const double rc = fmax(resid_l2 * resid_l2, 2.2250738585072014e-308);
+        m.bic[v] = zero_voxel ? 0.f : float(nn * log(rc / nn) + log(nn) * double(o.nnz));
+    }
 }
 
 __global__ void gather_i64_kernel(const int64_t* __restrict__ src, const int64_t* __restrict__ pos, int64_t n,
@@ -690,7 +723,7 @@ void launch_gram_t(const GdChunk& c, hipStream_t s) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(lds));
     hipLaunchKernelGGL(kern, dim3(unsigned(c.n)), dim3(64 * WAVES), lds, s, c.B, c.tab, c.data, c.sig_stride,
-                       c.field, c.vox, c.n, c.G, c.b);
+                       c.field, c.vox, c.n, c.G, c.b, c.sstat);
     SF_HIP(hipGetLastError());
 }
 
@@ -706,19 +739,6 @@ void launch_fista_t(const GdFistaArgs& a, int max_blocks, hipStream_t s) {
     if (max_blocks > 0) blocks = std::min<int64_t>(blocks, max_blocks);
     blocks = std::min<int64_t>(blocks, std::max<int64_t>(a.n, 1));
     hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(64 * W), 0, s, a);
-    SF_HIP(hipGetLastError());
-}
-
-template <int R>
-void launch_resid_t(const GdChunk& c, const GdOutArgs& o, hipStream_t s) {
-    constexpr int RC = 32, PQ = n_pairs_c(R) + n_ang_c(R);
-    const size_t lds = sizeof(double) * (size_t(6) * c.tab.N + size_t(RC) * PQ);
-    if (lds > 128 * 1024) throw Error{SHOREFIT_E_UNSUPPORTED, "gradient-deviation fit: too many gradients for LDS"};
-    auto kern = gd_residual_kernel<R, RC>;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              int(lds));
-    hipLaunchKernelGGL(kern, dim3(unsigned(c.n)), dim3(256), lds, s, c.B, c.tab, c.data, c.sig_stride, c.field,
-                       c.vox, c.n, o);
     SF_HIP(hipGetLastError());
 }
 
@@ -774,23 +794,18 @@ void gd_launch_fista(const GdVariant& v, const GdFistaArgs& a, int max_blocks, h
     throw Error{SHOREFIT_E_UNSUPPORTED, "no FISTA kernel for this variant"};
 }
 
-void gd_launch_residual(const GdChunk& c, const GdOutArgs& o, hipStream_t s) {
-    if (c.n == 0) return;
-    switch (c.B.R) {
-        case 2: return launch_resid_t<2>(c, o, s);
-        case 4: return launch_resid_t<4>(c, o, s);
-        case 6: return launch_resid_t<6>(c, o, s);
-        case 8: return launch_resid_t<8>(c, o, s);
-        case 10: return launch_resid_t<10>(c, o, s);
-        default: throw Error{SHOREFIT_E_UNSUPPORTED, "no residual kernel for this radial order"};
-    }
-}
-
 void gd_launch_identity(const float* field, const int64_t* vox, int64_t n, uint8_t* ident, uint8_t* devi,
                         hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(gd_identity_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, field, vox, n, ident,
                        devi);
+    SF_HIP(hipGetLastError());
+}
+
+void gd_launch_finish(const GdSlotOut* so, const double* sstat, const int64_t* vox, int64_t n, int N,
+                      const GdMaps& m, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(gd_finish_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, so, sstat, vox, n, N, m);
     SF_HIP(hipGetLastError());
 }
 

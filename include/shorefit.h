@@ -124,6 +124,8 @@ typedef struct ShorefitOutput {
     int64_t n_voxels;             /* masked voxels fitted */
     double lipschitz;             /* sigma_max(A)^2 / N used by FISTA */
     double fit_ms;                /* device time of the fit kernels */
+    double graddev_ms[3];         /* gradient-deviation fits: device time of the per-voxel Gram
+                                     kernels, of the FISTA kernels (with the residual), 0 (reserved) */
 } ShorefitOutput;
 
 /* Fill *spec with the cs-fit / ShoreModel defaults (src/fit.rs:647-669):
