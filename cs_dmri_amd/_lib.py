@@ -26,6 +26,8 @@ EXPORTS = (
     "shorefit_fista_batch", "shorefit_tikhonov_batch", "shorefit_last_error", "shorefit_version",
     "shorefit_tournier_matrix", "shorefit_odf_sh", "shorefit_predict", "shorefit_microstructure",
     "shorefit_fit_graddev", "shorefit_graddev_variants", "shorefit_graddev_tune", "shorefit_graddev_gram",
+    "shorefit_design_create", "shorefit_design_destroy", "shorefit_design_info", "shorefit_lasso_forward",
+    "shorefit_lasso_adjoint",
 )
 
 
@@ -133,6 +135,17 @@ def _load() -> C.CDLL:
     lib.shorefit_graddev_gram.argtypes = [_pg, _pv, _pdv, C.c_uint32, C.c_double, C.POINTER(C.c_int64), C.c_int64,
                                           _pf, _pd, _pd]
     lib.shorefit_graddev_gram.restype = C.c_int
+    lib.shorefit_design_create.argtypes = [_pd, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
+    lib.shorefit_design_create.restype = C.c_int
+    lib.shorefit_design_destroy.argtypes = [_vp]
+    lib.shorefit_design_destroy.restype = None
+    lib.shorefit_design_info.argtypes = [_vp, _pi, _pi, _pd]
+    lib.shorefit_design_info.restype = C.c_int
+    lib.shorefit_lasso_forward.argtypes = [_vp, _vp, C.c_int64, _vp, C.c_double, _vp, C.c_uint32, C.c_double,
+                                           C.c_int32, _vp, _vp, _vp]
+    lib.shorefit_lasso_forward.restype = C.c_int
+    lib.shorefit_lasso_adjoint.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp]
+    lib.shorefit_lasso_adjoint.restype = C.c_int
     return lib
 
 

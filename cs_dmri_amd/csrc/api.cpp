@@ -8,6 +8,7 @@
 #include <limits>
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -339,6 +340,7 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
     L.Mlow = ops.Mlow.p;
     L.Mlow2 = ops.Mlow2.p;
     L.step = float(ops.step);
+    L.step64 = ops.step;
     if (L.path_n > 0) {
         w.Xpath.alloc(size_t(V) * L.path_n * d.Kp);
         w.itspath.alloc(size_t(V) * L.path_n);
@@ -1002,6 +1004,13 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
 
 }  // namespace
 
+// A design held on the device between calls (shorefit_design_create).
+struct ShorefitDesign {
+    shorefit::Design d;
+    FistaOps ops;
+    FistaWork work;
+};
+
 extern "C" {
 
 const char* shorefit_last_error(void) { return g_last_error.c_str(); }
@@ -1100,6 +1109,92 @@ int shorefit_fista_batch(const double* A, int32_t n_rows, int32_t n_cols, const 
             if (converged) SF_HIP(hipMemcpy(converged, Cv.p, V, hipMemcpyDeviceToHost));
             if (residual_l2) SF_HIP(hipMemcpy(residual_l2, R.p, sizeof(float) * V, hipMemcpyDeviceToHost));
         }
+    });
+}
+
+int shorefit_design_create(const double* A, int32_t n_rows, int32_t n_cols, ShorefitDesign** out) {
+    return guarded([&] {
+        if (!A || !out || n_rows <= 0 || n_cols <= 0) throw Error{SHOREFIT_E_INVALID, "invalid design arguments"};
+        *out = nullptr;
+        auto h = std::make_unique<ShorefitDesign>();
+        hipStream_t s = nullptr;
+        upload_design(h->d, A, n_rows, n_cols, s);
+        FistaLaunch L{};
+        L.N = h->d.N;
+        L.K = h->d.K;
+        L.Kp = h->d.Kp;
+        L.Np = h->d.Np;
+        L.precision = SHOREFIT_PREC_FP32;
+        L.max_iter = 1;
+        setup_fista(h->d, h->ops, h->work, L, s);
+        SF_HIP(hipStreamSynchronize(s));
+        *out = h.release();
+    });
+}
+
+void shorefit_design_destroy(ShorefitDesign* design) { delete design; }
+
+int shorefit_design_info(const ShorefitDesign* h, int32_t* n_rows, int32_t* n_cols, double* lipschitz) {
+    return guarded([&] {
+        if (!h) throw Error{SHOREFIT_E_INVALID, "null design"};
+        if (n_rows) *n_rows = h->d.N;
+        if (n_cols) *n_cols = h->d.K;
+        if (lipschitz) *lipschitz = h->d.L;
+    });
+}
+
+int shorefit_lasso_forward(const ShorefitDesign* h, const float* signals, int64_t n_signals, const float* alpha,
+                           double alpha_scalar, const float* warm, uint32_t max_iter, double tol,
+                           int32_t non_negative, float* coef, uint32_t* iterations, void* stream) {
+    return guarded([&] {
+        if (!h || (n_signals > 0 && (!signals || !coef)) || n_signals < 0)
+            throw Error{SHOREFIT_E_INVALID, "invalid lasso_forward arguments"};
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        const Design& d = h->d;
+        const int64_t V = n_signals;
+        DevBuf<float> B(size_t(V) * d.Kp), mean(V);
+        FistaLaunch L{};
+        L.data = signals;
+        L.vox = nullptr;
+        L.n_vox = V;
+        L.N = d.N;
+        L.sig_stride = d.N;
+        L.K = d.K;
+        L.Kp = d.Kp;
+        L.Np = d.Np;
+        L.alpha = alpha_scalar;
+        L.max_iter = max_iter;
+        L.tol = float(tol);
+        L.nonneg = non_negative;
+        L.precision = SHOREFIT_PREC_FP32;
+        L.coef = coef;
+        L.iters = iterations;
+        L.A64p = h->ops.Ap.p;
+        L.c_dev = h->work.sc.p + 1;
+        L.Bbuf = B.p;
+        L.meanbuf = mean.p;
+        L.Mfrag = h->ops.M.p;
+        L.Afrag = h->ops.A.p;
+        L.step = float(h->ops.step);
+        L.step64 = h->ops.step;
+        L.thr = float(h->ops.step * alpha_scalar);
+        L.alpha_vox = alpha;
+        L.warm = warm;
+        launch_fista_rhs(L, s);
+        launch_fista(L, s);
+    });
+}
+
+int shorefit_lasso_adjoint(const ShorefitDesign* h, const float* coef, const float* grad_coef, int64_t n_signals,
+                           float* grad_signal, float* grad_alpha, uint8_t* status, void* stream) {
+    return guarded([&] {
+        if (!h || (n_signals > 0 && (!coef || !grad_coef)) || n_signals < 0)
+            throw Error{SHOREFIT_E_INVALID, "invalid lasso_adjoint arguments"};
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        launch_lasso_adjoint(h->d.G.p, h->d.A.p, h->d.N, h->d.K, coef, grad_coef, n_signals, grad_signal,
+                             grad_alpha, status, s);
     });
 }
 

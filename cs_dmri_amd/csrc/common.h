@@ -146,6 +146,9 @@ struct FistaLaunch {
     float* meanbuf;         // scratch [n_vox]: signal means
     float thr;              // step * alpha (fixed alpha)
     float step;             // 1 / L
+    double step64;          // 1 / L in FP64
+    const float* alpha_vox; // optional per-voxel alpha [n_vox] (FP32 kernel, fixed strategy)
+    const float* warm;      // optional warm start, indexed like coef (FP32 kernel)
     float ratio;            // alpha-ratio strategy: alpha = ratio * alpha_max (0 = fixed)
     double alpha;           // fixed alpha (for maps / BIC)
     uint32_t max_iter;
@@ -299,6 +302,11 @@ void gd_launch_finish(const GdSlotOut* so, const double* sstat, const int64_t* v
 void gd_launch_identity(const float* field, const int64_t* vox, int64_t n, uint8_t* ident, uint8_t* devi,
                         hipStream_t s);
 void launch_gather_i64(const int64_t* src, const int64_t* pos, int64_t n, int64_t* dst, hipStream_t s);
+
+// -------------------------------------------------------- differentiation ----
+size_t lasso_adjoint_lds(int K);
+void launch_lasso_adjoint(const double* G, const double* A, int N, int K, const float* coef, const float* gcoef,
+                          int64_t V, float* gsig, float* galpha, uint8_t* status, hipStream_t s);
 
 // ------------------------------------------------------------- post-fit ----
 // Per-mode weights of the microstructure scalars for one (radial order,

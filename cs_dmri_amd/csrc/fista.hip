@@ -64,6 +64,9 @@ struct FistaArgs {
     int Kp;
     float thr, step, ratio;
     double alpha;
+    double step64;             // 1 / L in FP64 (per-voxel thresholds)
+    const float* alpha_vox;    // optional per-voxel alpha, work order (FP32 kernel, fixed alpha)
+    const float* warm;         // optional warm start, indexed like coef (FP32 kernel)
     uint32_t max_iter;
     float tol;
     int nonneg;
@@ -283,6 +286,12 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
             alpha_v = double(bmax) / double(a.step) * double(a.ratio);
         }
     }
+    if (!PATH && a.alpha_vox) {
+        // per-voxel alpha (fixed strategy): thr_v = alpha_v / L
+        const float av = valid ? a.alpha_vox[v] : 0.f;
+        thr = float(a.step64 * double(av));
+        alpha_v = double(av);
+    }
     if (PATH && valid && zero_voxel && h == 0)
         for (int i = 0; i < a.path_n; ++i) {
             a.itspath[v * a.path_n + i] = 0;
@@ -292,6 +301,16 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_kernel(const FistaArgs a
     floatx4 x[MB], y[MB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) x[mb] = y[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (!PATH && a.warm && valid) {
+        // warm start x0 = y0 = warm, t = 1 (src/solver/fista.rs:76-80)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * mb + 4 * h + r;
+                x[mb][r] = y[mb][r] = k < a.K ? a.warm[out_idx * a.K + k] : 0.f;
+            }
+    }
     bool active = valid && !zero_voxel;
     bool conv = zero_voxel;
     uint32_t its = 0;
@@ -1137,6 +1156,9 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
     a.step = L.step;
     a.ratio = L.ratio;
     a.alpha = L.alpha;
+    a.step64 = L.step64;
+    a.alpha_vox = L.alpha_vox;
+    a.warm = L.warm;
     a.max_iter = L.max_iter;
     a.tol = L.tol;
     a.nonneg = L.nonneg;

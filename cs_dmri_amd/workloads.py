@@ -177,3 +177,16 @@ def config4(shape=(96, 96, 60), radial_order: int = 8):
     wl = Workload(f"shore{radial_order}-graddev-cartesian258-{'x'.join(map(str, shape))}", gt,
                   _volume(shape, mask, sig), mask, radial_order)
     return wl, field
+
+
+def config5(n_vox: int = 65536, frac: float = 0.1, seed: int = 6):
+    """Differentiation workload: config 2's table and signal generator
+    (sigma = S0/20) for n_vox voxels; frac of the diffusion-weighted
+    q-points held out (seed 6). Returns (table, signals (n_vox, N) float32,
+    train row indices, held-out row indices); the b0 row is always trained."""
+    gt, sig = config2_signals(n_vox)
+    rng = np.random.default_rng(seed)
+    dw = np.flatnonzero(gt.bvals > gt.b0_threshold)
+    held = np.sort(rng.choice(dw, size=int(round(frac * len(dw))), replace=False))
+    train = np.setdiff1d(np.arange(len(gt.bvals)), held)
+    return gt, sig, train, held

@@ -192,6 +192,36 @@ int shorefit_graddev_gram(const ShorefitGradients* grads, const ShorefitVolume* 
                           uint32_t radial_order, double zeta, const int64_t* voxels, int64_t n_voxels,
                           float* gram_out, double* rhs_out, double* lipschitz_out);
 
+/* ------------------------------------------------------ differentiation ----
+ * The reference has no derivative of the fit; these entry points are new.
+ * A design kept on the device between calls (its FP64 copy, A^T A, the
+ * Lipschitz constant and the FP32 kernel operands): the set-up of
+ * FistaSolver::new (src/solver/fista.rs:38-62) done once. */
+typedef struct ShorefitDesign ShorefitDesign;
+
+/* A: host, n_rows x n_cols row-major float64. */
+int shorefit_design_create(const double* A, int32_t n_rows, int32_t n_cols, ShorefitDesign** out);
+void shorefit_design_destroy(ShorefitDesign* design);
+int shorefit_design_info(const ShorefitDesign* design, int32_t* n_rows, int32_t* n_cols, double* lipschitz);
+
+/* FistaSolver fits (src/solver/fista.rs:70-143) of V signals on the device
+ * with a per-signal alpha (alpha: V float32, or NULL for alpha_scalar) and
+ * an optional warm start (warm: V x K float32, x0 = y0 = warm, t = 1, as at
+ * src/solver/fista.rs:76-80; may alias coef). FP32 kernel. Device pointers:
+ * signals V x N, coef V x K out, iterations V (may be NULL). */
+int shorefit_lasso_forward(const ShorefitDesign* design, const float* signals, int64_t n_signals, const float* alpha,
+                           double alpha_scalar, const float* warm, uint32_t max_iter, double tol,
+                           int32_t non_negative, float* coef, uint32_t* iterations, void* stream);
+
+/* Vector-Jacobian product of the lasso solution by implicit differentiation
+ * on each signal's active set S = {k : coef_k != 0}, z = sign(coef_S): with
+ * G = A^T A, w = G_SS^-1 grad_coef_S (FP64 Cholesky), grad_signal = A_S w
+ * (V x N) and grad_alpha = -N z^T w (V). Device pointers; grad_signal and
+ * grad_alpha may be NULL; status (V, may be NULL) is 0, or 1 when G_SS is not
+ * positive definite (the signal's gradients are then NaN). K <= 180. */
+int shorefit_lasso_adjoint(const ShorefitDesign* design, const float* coef, const float* grad_coef,
+                           int64_t n_signals, float* grad_signal, float* grad_alpha, uint8_t* status, void* stream);
+
 /* Number of SHORE coefficients at radial_order (ShoreBasis::n_coeffs,
  * src/basis/shore.rs:29-44, :118-120): 72 / 145 / 256 at order 6 / 8 / 10.
  * Returns -1 for an odd order. */

@@ -73,6 +73,8 @@ class Oracle:
                                     C.c_uint32, _d, _i, _pd, _pd, _d, _d, _i, _pf, _pf, _pf, _pu32, _pu8,
                                     _pf, _pf, _pf, _pd]),
             "orc_effective_table": (None, [_pd, _pd, _pd, _pd, _pd, _i, _d, _pd, _pd]),
+            "orc_lasso_adjoint": (_i, [_pd, _i, _i, _pd, _pd, _i, _pd, _pd]),
+            "orc_lasso_polish": (_i, [_pd, _i, _i, _pd, _d, _pd, _pd]),
             "orc_fit_graddev": (_i, [_pf, _pu8, _i, _i, _i, _i, _pd, _pd, _pd, _pd, _d, _d, _d, _u, _d, _pf,
                                      _i, _d, _d, C.c_uint32, _d, _i, _i, _pf, _pf, _pf, _pu32, _pu8, _pf, _pf,
                                      _pd]),
@@ -293,6 +295,29 @@ class Oracle:
         if rc != 0:
             raise ValueError("graddev oracle supports alpha_mode fixed and alpha-ratio only")
         return out
+
+    # -- differentiation (no reference counterpart) ----------------------
+    def lasso_adjoint(self, A, coef, grad):
+        """(grad_signal (V, N), grad_alpha (V,)) of the active-set implicit
+        derivative, FP64."""
+        A = _f64(A)
+        N, K = A.shape
+        c = _f64(np.atleast_2d(coef))
+        g = _f64(np.atleast_2d(grad))
+        V = c.shape[0]
+        gs = np.zeros((V, N))
+        ga = np.zeros(V)
+        self.lib.orc_lasso_adjoint(_ptr(A, _pd), N, K, _ptr(c, _pd), _ptr(g, _pd), V, _ptr(gs, _pd), _ptr(ga, _pd))
+        return gs, ga
+
+    def lasso_polish(self, A, s, alpha, coef):
+        """Exact minimiser on coef's active set and signs; (c, status), status
+        0 when KKT-verified."""
+        A, s, c = _f64(A), _f64(s), _f64(coef)
+        N, K = A.shape
+        out = np.zeros(K)
+        st = self.lib.orc_lasso_polish(_ptr(A, _pd), N, K, _ptr(s, _pd), float(alpha), _ptr(c, _pd), _ptr(out, _pd))
+        return out, st
 
     # -- post-fit (src/odf.rs, src/scalars/microstructure.rs, src/synth.rs) --
     def hyp2f1(self, a, b, c, z) -> float:

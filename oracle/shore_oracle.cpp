@@ -1245,4 +1245,111 @@ void orc_project(const double* M, int rows, int K, const float* coefs, int64_t V
         }
 }
 
+// ------------------------------------------------------ differentiation ----
+// No counterpart in the reference. For the lasso objective of
+// src/solver/fista.rs (1/2N |s - A c|^2 + alpha |c|_1), the optimality
+// conditions on the active set S (c_S != 0, z = sign(c_S)) are
+// G_SS c_S = A_S^T s - N alpha z with G = A^T A. The textbook consequence:
+// w = G_SS^-1 g_S, dL/ds = A_S w, dL/dalpha = -N z^T w for an upstream
+// gradient g. FP64 Cholesky. Returns the number of signals whose G_SS is not
+// positive definite (their outputs are NaN).
+namespace {
+bool cholesky_solve(std::vector<double>& M, int m, std::vector<double>& x) {
+    for (int j = 0; j < m; ++j) {
+        double d = M[size_t(j) * m + j];
+        for (int k = 0; k < j; ++k) d -= M[size_t(j) * m + k] * M[size_t(j) * m + k];
+        if (!(d > 0.0)) return false;
+        const double ljj = std::sqrt(d);
+        M[size_t(j) * m + j] = ljj;
+        for (int i = j + 1; i < m; ++i) {
+            double v = M[size_t(i) * m + j];
+            for (int k = 0; k < j; ++k) v -= M[size_t(i) * m + k] * M[size_t(j) * m + k];
+            M[size_t(i) * m + j] = v / ljj;
+        }
+    }
+    for (int j = 0; j < m; ++j) {
+        double v = x[j];
+        for (int k = 0; k < j; ++k) v -= M[size_t(j) * m + k] * x[k];
+        x[j] = v / M[size_t(j) * m + j];
+    }
+    for (int j = m - 1; j >= 0; --j) {
+        double v = x[j];
+        for (int k = j + 1; k < m; ++k) v -= M[size_t(k) * m + j] * x[k];
+        x[j] = v / M[size_t(j) * m + j];
+    }
+    return true;
+}
+}  // namespace
+
+int orc_lasso_adjoint(const double* A, int N, int K, const double* coef, const double* grad, int V, double* grad_s,
+                      double* grad_alpha) {
+    const std::vector<double> G = gram(A, N, K);
+    int failed = 0;
+    for (int v = 0; v < V; ++v) {
+        const double* c = coef + size_t(v) * K;
+        std::vector<int> S;
+        for (int k = 0; k < K; ++k)
+            if (c[k] != 0.0) S.push_back(k);
+        const int m = int(S.size());
+        std::vector<double> M(size_t(m) * m), w(m);
+        for (int i = 0; i < m; ++i) {
+            w[i] = grad[size_t(v) * K + S[i]];
+            for (int j = 0; j < m; ++j) M[size_t(i) * m + j] = G[size_t(S[i]) * K + S[j]];
+        }
+        if (m > 0 && !cholesky_solve(M, m, w)) {
+            ++failed;
+            for (int n = 0; n < N; ++n) grad_s[size_t(v) * N + n] = std::nan("");
+            grad_alpha[v] = std::nan("");
+            continue;
+        }
+        double zw = 0.0;
+        for (int i = 0; i < m; ++i) zw += (c[S[i]] > 0.0 ? w[i] : -w[i]);
+        grad_alpha[v] = -double(N) * zw;
+        for (int n = 0; n < N; ++n) {
+            double acc = 0.0;
+            for (int i = 0; i < m; ++i) acc += A[size_t(n) * K + S[i]] * w[i];
+            grad_s[size_t(v) * N + n] = acc;
+        }
+    }
+    return failed;
+}
+
+// The exact lasso minimiser on the active set and signs of c_in: c_S solves
+// G_SS c_S = A_S^T s - N alpha z. Returns 0 when it is verified optimal
+// (signs kept, |A_k^T (s - A c)| / N <= alpha (1 + 1e-9) off S), 1 on a sign
+// change, 2 on a violated inactive condition, 3 if G_SS is not PD.
+int orc_lasso_polish(const double* A, int N, int K, const double* s, double alpha, const double* c_in,
+                     double* c_out) {
+    const std::vector<double> G = gram(A, N, K);
+    std::vector<int> S;
+    for (int k = 0; k < K; ++k)
+        if (c_in[k] != 0.0) S.push_back(k);
+    const int m = int(S.size());
+    std::vector<double> M(size_t(m) * m), x(m);
+    for (int i = 0; i < m; ++i) {
+        double ats = 0.0;
+        for (int n = 0; n < N; ++n) ats += A[size_t(n) * K + S[i]] * s[n];
+        x[i] = ats - double(N) * alpha * (c_in[S[i]] > 0.0 ? 1.0 : -1.0);
+        for (int j = 0; j < m; ++j) M[size_t(i) * m + j] = G[size_t(S[i]) * K + S[j]];
+    }
+    if (m > 0 && !cholesky_solve(M, m, x)) return 3;
+    std::fill(c_out, c_out + K, 0.0);
+    for (int i = 0; i < m; ++i) c_out[S[i]] = x[i];
+    for (int i = 0; i < m; ++i)
+        if ((x[i] > 0.0) != (c_in[S[i]] > 0.0)) return 1;
+    std::vector<double> r(N);
+    for (int n = 0; n < N; ++n) {
+        double acc = 0.0;
+        for (int k = 0; k < K; ++k) acc += A[size_t(n) * K + k] * c_out[k];
+        r[n] = s[n] - acc;
+    }
+    for (int k = 0; k < K; ++k) {
+        if (c_out[k] != 0.0) continue;
+        double g = 0.0;
+        for (int n = 0; n < N; ++n) g += A[size_t(n) * K + k] * r[n];
+        if (std::fabs(g) / double(N) > alpha * (1.0 + 1e-9)) return 2;
+    }
+    return 0;
+}
+
 }  // extern "C"
