@@ -9,10 +9,10 @@
 // For an upstream gradient g = dL/dc the vector-Jacobian products are
 //     w = G_SS^-1 g_S ,   dL/ds = A_S w ,   dL/dalpha = -N z^T w .
 //
-// One workgroup per signal: the active set by a block-wide compaction, G_SS
-// gathered from the FP64 Gram matrix into a packed lower triangle in LDS, an
-// in-place right-looking Cholesky (FP64), the two triangular solves, then
-// A_S w with A read from L2.
+// One wave per signal: the active set by ballots, G_SS gathered from the FP64
+// Gram matrix into a packed lower triangle in LDS (sized by the batch's
+// largest active set), an in-place right-looking Cholesky (FP64), the two
+// triangular solves, then A_S w with A read from L2.
 #include "common.h"
 
 namespace shorefit {
@@ -35,7 +35,10 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
-constexpr int kThreads = 256;
+// One wave per signal: the workgroup barriers below synchronise a single
+// wave, so they cost next to nothing, and the LDS per signal is sized by the
+// batch's largest active set, so several signals share a CU.
+constexpr int kThreads = 64;
 
 __global__ void __launch_bounds__(kThreads)
     lasso_adjoint_kernel(const double* __restrict__ G, const double* __restrict__ A, int N, int K,
@@ -43,32 +46,27 @@ __global__ void __launch_bounds__(kThreads)
                          float* __restrict__ gsig, float* __restrict__ galpha, uint8_t* __restrict__ status) {
     extern __shared__ double sm[];
     __shared__ int s_idx[256];
-    __shared__ int s_cnt[kThreads / 64];
     __shared__ int s_fail;
-    __shared__ double s_red[kThreads / 64];
     const int64_t v = blockIdx.x;
     if (v >= V) return;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lane = threadIdx.x;
     const float* c = coef + v * K;
 
-    // active set S in coefficient order (K <= 256: one coefficient per thread)
-    const bool act = tid < K && c[tid] != 0.f;
-    const uint64_t bal = __ballot(act);
-    if (lane == 0) s_cnt[w] = __popcll(bal);
-    if (tid == 0) s_fail = 0;
-    __syncthreads();
-    int base = 0, m = 0;
-    for (int q = 0; q < kThreads / 64; ++q) {
-        if (q < w) base += s_cnt[q];
-        m += s_cnt[q];
+    // active set S in coefficient order
+    int m = 0;
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        const bool act = k0 + lane < K && c[k0 + lane] != 0.f;
+        const uint64_t bal = __ballot(act);
+        if (act) s_idx[m + __popcll(bal & ((uint64_t(1) << lane) - 1))] = k0 + lane;
+        m += __popcll(bal);
     }
-    if (act) s_idx[base + __popcll(bal & ((uint64_t(1) << lane) - 1))] = tid;
+    if (lane == 0) s_fail = 0;
     __syncthreads();
 
     if (m == 0) {  // c = 0 is locally constant: zero gradients
         if (gsig)
-            for (int n = tid; n < N; n += kThreads) gsig[v * N + n] = 0.f;
-        if (tid == 0) {
+            for (int n = lane; n < N; n += kThreads) gsig[v * N + n] = 0.f;
+        if (lane == 0) {
             if (galpha) galpha[v] = 0.f;
             if (status) status[v] = 0;
         }
@@ -77,18 +75,18 @@ __global__ void __launch_bounds__(kThreads)
     double* L = sm;                // packed lower triangle of G_SS, m (m + 1) / 2
     double* x = sm + tri(m);       // right-hand side, then the solution w
     const int nt = tri(m);
-    for (int e = tid; e < nt; e += kThreads) {
+    for (int e = lane; e < nt; e += kThreads) {
         int i, j;
         tri_decode(e, i, j);
         L[e] = G[size_t(s_idx[i]) * K + s_idx[j]];
     }
-    for (int i = tid; i < m; i += kThreads) x[i] = double(gcoef[v * K + s_idx[i]]);
+    for (int i = lane; i < m; i += kThreads) x[i] = double(gcoef[v * K + s_idx[i]]);
     __syncthreads();
 
     // Cholesky G_SS = L L^T, column by column
     for (int j = 0; j < m; ++j) {
         const int jj = tri(j) + j;
-        if (tid == 0) {
+        if (lane == 0) {
             const double d = L[jj];
             if (!(d > 0.0)) s_fail = 1;
             L[jj] = sqrt(d);
@@ -96,10 +94,10 @@ __global__ void __launch_bounds__(kThreads)
         __syncthreads();
         if (s_fail) break;
         const double ljj = L[jj];
-        for (int i = j + 1 + tid; i < m; i += kThreads) L[tri(i) + j] /= ljj;
+        for (int i = j + 1 + lane; i < m; i += kThreads) L[tri(i) + j] /= ljj;
         __syncthreads();
         const int n = m - j - 1, cnt = tri(n);
-        for (int e = tid; e < cnt; e += kThreads) {
+        for (int e = lane; e < cnt; e += kThreads) {
             int ii, kk;
             tri_decode(e, ii, kk);
             const int i = j + 1 + ii, k = j + 1 + kk;
@@ -110,8 +108,8 @@ __global__ void __launch_bounds__(kThreads)
     if (s_fail) {
         const float nan = __int_as_float(0x7fc00000);
         if (gsig)
-            for (int n = tid; n < N; n += kThreads) gsig[v * N + n] = nan;
-        if (tid == 0) {
+            for (int n = lane; n < N; n += kThreads) gsig[v * N + n] = nan;
+        if (lane == 0) {
             if (galpha) galpha[v] = nan;
             if (status) status[v] = 1;
         }
@@ -119,56 +117,75 @@ __global__ void __launch_bounds__(kThreads)
     }
     // L y = g, then L^T w = y
     for (int j = 0; j < m; ++j) {
-        if (tid == 0) x[j] /= L[tri(j) + j];
+        if (lane == 0) x[j] /= L[tri(j) + j];
         __syncthreads();
         const double xj = x[j];
-        for (int i = j + 1 + tid; i < m; i += kThreads) x[i] -= L[tri(i) + j] * xj;
+        for (int i = j + 1 + lane; i < m; i += kThreads) x[i] -= L[tri(i) + j] * xj;
         __syncthreads();
     }
     for (int j = m - 1; j >= 0; --j) {
-        if (tid == 0) x[j] /= L[tri(j) + j];
+        if (lane == 0) x[j] /= L[tri(j) + j];
         __syncthreads();
         const double xj = x[j];
-        for (int i = tid; i < j; i += kThreads) x[i] -= L[tri(j) + i] * xj;
+        for (int i = lane; i < j; i += kThreads) x[i] -= L[tri(j) + i] * xj;
         __syncthreads();
     }
 
     // dL/dalpha = -N z^T w
     double zw = 0.0;
-    for (int i = tid; i < m; i += kThreads) zw += (c[s_idx[i]] > 0.f ? x[i] : -x[i]);
+    for (int i = lane; i < m; i += kThreads) zw += (c[s_idx[i]] > 0.f ? x[i] : -x[i]);
     zw = wave_sum(zw);
-    if (lane == 0) s_red[w] = zw;
     // dL/ds = A_S w
     if (gsig)
-        for (int n = tid; n < N; n += kThreads) {
+        for (int n = lane; n < N; n += kThreads) {
             const double* arow = A + size_t(n) * K;
             double acc = 0.0;
             for (int i = 0; i < m; ++i) acc = fma(arow[s_idx[i]], x[i], acc);
             gsig[v * N + n] = float(acc);
         }
-    __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int q = 0; q < kThreads / 64; ++q) t += s_red[q];
-        if (galpha) galpha[v] = float(-double(N) * t);
+    if (lane == 0) {
+        if (galpha) galpha[v] = float(-double(N) * zw);
         if (status) status[v] = 0;
     }
 }
 
+// Largest active-set size of the batch (atomic max over the signals).
+__global__ void active_max_kernel(const float* __restrict__ coef, int K, int64_t V, int* __restrict__ out) {
+    const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int m = 0;
+    if (v < V)
+        for (int k = 0; k < K; ++k) m += coef[v * K + k] != 0.f;
+    m = max(m, __shfl_xor(m, 1));
+    m = max(m, __shfl_xor(m, 2));
+    m = max(m, __shfl_xor(m, 4));
+    m = max(m, __shfl_xor(m, 8));
+    m = max(m, __shfl_xor(m, 16));
+    m = max(m, __shfl_xor(m, 32));
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
 }  // namespace
 
-size_t lasso_adjoint_lds(int K) { return sizeof(double) * (size_t(K) * (K + 1) / 2 + K); }
+size_t lasso_adjoint_lds(int m) { return sizeof(double) * (size_t(m) * (m + 1) / 2 + m); }
 
 void launch_lasso_adjoint(const double* G, const double* A, int N, int K, const float* coef, const float* gcoef,
                           int64_t V, float* gsig, float* galpha, uint8_t* status, hipStream_t s) {
     if (V <= 0) return;
     if (K > 256) throw Error{SHOREFIT_E_UNSUPPORTED, "lasso adjoint: at most 256 coefficients"};
-    const size_t lds = lasso_adjoint_lds(K);
+    DevBuf<int> d_m(1);
+    SF_HIP(hipMemsetAsync(d_m.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(active_max_kernel, dim3(unsigned((V + 255) / 256)), dim3(256), 0, s, coef, K, V, d_m.p);
+    SF_HIP(hipGetLastError());
+    int m_max = 0;
+    SF_HIP(hipMemcpyAsync(&m_max, d_m.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SF_HIP(hipStreamSynchronize(s));
+    const size_t lds = lasso_adjoint_lds(std::max(m_max, 1));
     if (lds > 150 * 1024)
-        throw Error{SHOREFIT_E_UNSUPPORTED,
-                    "lasso adjoint: the packed Gram matrix of " + std::to_string(K) + " coefficients exceeds LDS"};
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(lasso_adjoint_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        throw Error{SHOREFIT_E_UNSUPPORTED, "lasso adjoint: an active set of " + std::to_string(m_max) +
+                                                " coefficients exceeds LDS (at most 190)"};
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(lasso_adjoint_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     hipLaunchKernelGGL(lasso_adjoint_kernel, dim3(unsigned(V)), dim3(kThreads), lds, s, G, A, N, K, coef, gcoef, V,
                        gsig, galpha, status);
     SF_HIP(hipGetLastError());

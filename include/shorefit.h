@@ -218,7 +218,8 @@ int shorefit_lasso_forward(const ShorefitDesign* design, const float* signals, i
  * G = A^T A, w = G_SS^-1 grad_coef_S (FP64 Cholesky), grad_signal = A_S w
  * (V x N) and grad_alpha = -N z^T w (V). Device pointers; grad_signal and
  * grad_alpha may be NULL; status (V, may be NULL) is 0, or 1 when G_SS is not
- * positive definite (the signal's gradients are then NaN). K <= 180. */
+ * positive definite (the signal's gradients are then NaN). Active sets of at
+ * most 190 coefficients. */
 int shorefit_lasso_adjoint(const ShorefitDesign* design, const float* coef, const float* grad_coef,
                            int64_t n_signals, float* grad_signal, float* grad_alpha, uint8_t* status, void* stream);
 
