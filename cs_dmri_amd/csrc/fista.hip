@@ -18,6 +18,8 @@
 // {16kb + 4h + r : h}), so Y never leaves registers. M is staged once per
 // workgroup in LDS in "fragment-major" order (one float4 per lane per 16x16
 // block, read with conflict-free ds_read_b128).
+#include <type_traits>
+
 #include "common.h"
 
 namespace shorefit {
@@ -593,6 +595,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     __syncthreads();
 
     floatx4 x[MB], y[MB], acc[MB], breg[MB];
+    floatx4 x2[MB];  // the other register set of x in the inner loop
     int64_t vid = -1;
     uint32_t it = 0;
     float thr_u = a.thr, thr = a.thr, sc = 1.f, mprev = 0.f, dprev = 0.f, mom = 0.f;
@@ -664,14 +667,38 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     // |x| of every slot's voxel (FP32 sum of squares in a fixed order).
     // The opaque start value keeps the compiler from merging the two uses
     // and hoisting the sum onto the every-iteration path.
-    auto xnorm = [&]() {
+    auto xnorm = [&](const floatx4(&xv)[MB]) {
         float q = 0.f;
         asm volatile("" : "+v"(q));
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) q = fmaf(x[mb][r], x[mb][r], q);
+            for (int r = 0; r < 4; ++r) q = fmaf(xv[mb][r], xv[mb][r], q);
         return sqrtf(lane_sum4(q));
+    };
+
+    // Writes out a finished slot (or, PATH, its last path point).
+    auto write_out = [&](const floatx4(&xv)[MB], bool cv) {
+        const float isc = 1.f / sc;  // exact: sc is a power of two
+        if constexpr (PATH) {
+            const int64_t pt = vid * a.path_n + pi;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+                *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = xv[mb] * isc;
+            if (h == 0) {
+                a.itspath[pt] = zero ? 0u : it;
+                a.convpath[pt] = uint8_t((cv ? 1 : 0) | (zero ? 2 : 0));
+            }
+        } else {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+                *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = xv[mb] * isc;
+            if (h == 0) {
+                a.its[vid] = zero ? 0u : it;
+                a.status[vid] = uint8_t((cv ? 1 : 0) | (zero ? 2 : 0));
+                a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
+            }
+        }
     };
 
     uint32_t wit = 0;
@@ -682,8 +709,14 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     // per-iteration path.
     for (;;) {
         if (__ballot(alive) == 0ull) break;
-        bool done = false, conv = false;
-        for (;;) {
+        bool done = false, conv = false, wrote = false;
+        // One FISTA iteration from xi to xo (y and acc in place). The inner
+        // loop runs two per trip and alternates the two register sets of x,
+        // so x_new, which is live together with x until d = x_new - x, is
+        // never copied over x. A slot that finishes in the first half is
+        // written out there and sits out the second half (its lanes compute
+        // one discarded iteration); the trip then ends and the slots refill.
+        auto step = [&](floatx4(&xi)[MB], floatx4(&xo)[MB], auto first) __attribute__((always_inline)) -> bool {
             // An opaque zero offset keeps the (loop-invariant) fragment loads
             // inside the loop instead of hoisted into registers for all blocks.
             int moff = 0;
@@ -705,11 +738,11 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                         const float val = y[mb][r] + acc[mb][r];
                         float xn = val - __builtin_amdgcn_fmed3f(val, -thr, thr);  // strict soft threshold
                         if constexpr (NONNEG) xn = fmaxf(xn, 0.f);
-                        const float d = xn - x[mb][r];
+                        const float d = xn - xi[mb][r];
                         const float yn = fmaf(mom, d, xn);
                         dd = fmaf(d, d, dd);
                         t[4 * s2 + r] = yn - y[mb][r];
-                        x[mb][r] = xn;
+                        xo[mb][r] = xn;
                         y[mb][r] = yn;
                     }
 #pragma unroll
@@ -734,7 +767,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
             bool cv = nd < tol * lo * (1.f - 1e-6f);
             const bool amb = !cv && !(nd > tol * hi * (1.f + 1e-6f));
             if (__ballot(alive && amb) != 0ull) {
-                const float xs = xnorm();
+                const float xs = xnorm(xo);
                 if (amb) cv = nd / fmaxf(xs, fl) < tol;
                 xref = xs;
                 drift = 0.f;
@@ -754,7 +787,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                         const int64_t pt = vid * a.path_n + pi;
 #pragma unroll
                         for (int mb = 0; mb < MB; ++mb)
-                            *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
+                            *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = xo[mb] * isc;
                         if (h == 0) {
                             a.itspath[pt] = zero ? 0u : it;
                             a.convpath[pt] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
@@ -773,13 +806,13 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                         for (int q = 0; q < 4; ++q) {
                             const int mb = 2 * kc + (q >> 1), r0 = 2 * (q & 1);
                             float2v t2 = unpack2<BF16>(dyh[kc][q]);
-                            t2[0] -= y[mb][r0] - x[mb][r0];
-                            t2[1] -= y[mb][r0 + 1] - x[mb][r0 + 1];
+                            t2[0] -= y[mb][r0] - xo[mb][r0];
+                            t2[1] -= y[mb][r0 + 1] - xo[mb][r0 + 1];
                             if (next) dyh[kc][q] = pack2<BF16>(t2[0], t2[1]);
                         }
 #pragma unroll
                     for (int mb = 0; mb < MB; ++mb)
-                        if (next) y[mb] = x[mb];
+                        if (next) y[mb] = xo[mb];
                 }
             }
             const float bound = (1.f + mom) * nd + mprev * dprev;
@@ -807,11 +840,11 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                         glo[i / KC][i % KC] = *reinterpret_cast<const uint4*>(lo16 + (loff + uint32_t(i) * 1024u));
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                const float xs = xnorm();
+                const float xs = xnorm(xo);
                 const float f = fminf(pow2_to(bound, 10), pow2_to(xs, 40));
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb) {
-                    x[mb] *= f;
+                    xo[mb] *= f;
                     y[mb] *= f;
                 }
                 sc *= f;
@@ -918,30 +951,24 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                 }
             }
             mom = mom_next;
-            if (__ballot(done) != 0ull) break;
-        }
-        if (done) {
-            const float isc = 1.f / sc;  // exact: sc is a power of two
-            if constexpr (PATH) {
-                const int64_t pt = vid * a.path_n + pi;
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-                    *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
-                if (h == 0) {
-                    a.itspath[pt] = zero ? 0u : it;
-                    a.convpath[pt] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
+            if constexpr (decltype(first)::value) {
+                if (__ballot(done) != 0ull && done) {
+                    write_out(xo, conv);
+                    wrote = true;
+                    alive = false;  // until the refill: no iteration counts, no ballots
                 }
+                return false;
             } else {
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-                    *reinterpret_cast<floatx4*>(a.X + vid * a.Kp + 16 * mb + 4 * h) = x[mb] * isc;
-                if (h == 0) {
-                    a.its[vid] = zero ? 0u : it;
-                    a.status[vid] = uint8_t((conv ? 1 : 0) | (zero ? 2 : 0));
-                    a.alpha_v[vid] = zero ? 0.f : (a.ratio > 0.f ? thr_u / a.step : a.alpha_fixed);
-                }
+                return __ballot(done) != 0ull;
             }
+        };
+        // The loop exits only after a second half, so no path out of it
+        // leaves x in the other register set (which would need a copy).
+        for (;;) {
+            (void)step(x, x2, std::true_type{});
+            if (step(x2, x, std::false_type{})) break;
         }
+        if (done && !wrote) write_out(x, conv);
         refill(done);
     }
 }
