@@ -193,27 +193,36 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
 // FP64 accumulation: b enters every FISTA iteration, so its rounding is a
 // perturbation of the problem itself; accumulating the N-term dot products
 // in FP32 measurably moved the FP32 trajectories away from the FP64 ones.
-// Workgroup = 64 voxels x 4 waves; the 64 lanes are the voxels (signal
-// chunks staged in LDS) and wave w owns the contiguous coefficient range
-// [w KQ, (w+1) KQ) of the zero-padded design Ap (N x Kp, FP64), which it reads
-// with wave-uniform (scalar) loads.
-template <int KQ>  // KQ = Kp / 4 coefficients per wave
-__global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data, const int64_t* __restrict__ vox,
+// FP64 FMAs on the vector ALUs: on gfx950 they outrun the FP64 MFMA (65 vs
+// 49 TFLOP/s measured), and an LDS-broadcast variant of this kernel ran 1.6x
+// slower than these scalar loads.
+// Workgroup = 8 waves x 64 lanes x VPL voxels per lane (signal chunks staged
+// in LDS); wave w owns the contiguous coefficient range [w KQ, (w+1) KQ) of
+// the zero-padded design Ap (N x Kp, FP64), which it reads with wave-uniform
+// (scalar) loads. Each loaded element feeds VPL FMAs per lane.
+template <int KQ, int VPL>  // KQ = Kp / 8 coefficients per wave
+__global__ void __launch_bounds__(512) rhs_kernel(const float* __restrict__ data, const int64_t* __restrict__ vox,
                                                   int64_t n_vox, int64_t sig_stride, int N, int Kp,
                                                   const double* __restrict__ Ap, const double* __restrict__ cp,
                                                   float* __restrict__ B, float* __restrict__ mean) {
-    constexpr int CH = 64;  // samples per LDS chunk
-    __shared__ float sm[64][CH + 1];
+    static_assert(KQ % 2 == 0, "coefficients are stored in pairs");
+    constexpr int CH = 32;        // samples per LDS chunk
+    constexpr int NV = 64 * VPL;  // voxels per workgroup
+    __shared__ float sm[NV][CH + 1];
     const int lane = threadIdx.x & 63;
     const int k0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * KQ;
-    const int64_t v0 = int64_t(blockIdx.x) * 64;
-    double acc[KQ];
+    const int64_t v0 = int64_t(blockIdx.x) * NV;
+    double acc[VPL][KQ];
+    double sum[VPL];
 #pragma unroll
-    for (int q = 0; q < KQ; ++q) acc[q] = 0.0;
-    double sum = 0.0;
+    for (int i = 0; i < VPL; ++i) {
+        sum[i] = 0.0;
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) acc[i][q] = 0.0;
+    }
     for (int n0 = 0; n0 < N; n0 += CH) {
         __syncthreads();
-        for (int t = threadIdx.x; t < 64 * CH; t += 256) {
+        for (int t = threadIdx.x; t < NV * CH; t += 512) {
             const int vj = t / CH, nn = t % CH;
             const int64_t v = v0 + vj;
             float x = 0.f;
@@ -223,21 +232,31 @@ __global__ void __launch_bounds__(256) rhs_kernel(const float* __restrict__ data
         __syncthreads();
         const int nmax = min(CH, N - n0);
         for (int nn = 0; nn < nmax; ++nn) {
-            const double sv = double(sm[lane][nn]);
-            sum += sv;
+            double sv[VPL];
+#pragma unroll
+            for (int i = 0; i < VPL; ++i) {
+                sv[i] = double(sm[lane + 64 * i][nn]);
+                sum[i] += sv[i];
+            }
             const double* arow = Ap + size_t(n0 + nn) * Kp + k0;
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) acc[q] = fma(arow[q], sv, acc[q]);
+            for (int q = 0; q < KQ; ++q) {
+                const double aq = arow[q];
+#pragma unroll
+                for (int i = 0; i < VPL; ++i) acc[i][q] = fma(aq, sv[i], acc[i][q]);
+            }
         }
     }
-    const int64_t v = v0 + lane;
-    if (v >= n_vox) return;
     const double c = *cp;  // step / N, computed on the device from the Lipschitz constant
 #pragma unroll
-    for (int q = 0; q < KQ; q += 4)
-        *reinterpret_cast<floatx4*>(B + v * Kp + k0 + q) =
-            floatx4{float(c * acc[q]), float(c * acc[q + 1]), float(c * acc[q + 2]), float(c * acc[q + 3])};
-    if (k0 == 0) mean[v] = float(sum / double(N));
+    for (int i = 0; i < VPL; ++i) {
+        const int64_t v = v0 + lane + 64 * i;
+        if (v >= n_vox) continue;
+#pragma unroll
+        for (int q = 0; q < KQ; q += 2)
+            *reinterpret_cast<float2*>(B + v * Kp + k0 + q) = make_float2(float(c * acc[i][q]), float(c * acc[i][q + 1]));
+        if (k0 == 0) mean[v] = float(sum[i] / double(N));
+    }
 }
 
 // Plain FP32 form: V = M Y + B recomputed exactly on FP32 MFMA every
@@ -463,9 +482,10 @@ void launch_tik(const FistaArgs& args, int64_t n_vox, float* rss_l2, hipStream_t
 
 template <int MB>
 void launch_rhs(const FistaLaunch& L, hipStream_t s) {
-    const unsigned grid = unsigned((L.n_vox + 63) / 64);
-    hipLaunchKernelGGL(rhs_kernel<4 * MB>, dim3(grid), dim3(256), 0, s, L.data, L.vox, L.n_vox, L.sig_stride, L.N,
-                       L.Kp, L.A64p, L.c_dev, L.Bbuf, L.meanbuf);
+    constexpr int VPL = 2;
+    const unsigned grid = unsigned((L.n_vox + 64 * VPL - 1) / (64 * VPL));
+    hipLaunchKernelGGL((rhs_kernel<2 * MB, VPL>), dim3(grid), dim3(512), 0, s, L.data, L.vox, L.n_vox, L.sig_stride,
+                       L.N, L.Kp, L.A64p, L.c_dev, L.Bbuf, L.meanbuf);
     SF_HIP(hipGetLastError());
 }
 

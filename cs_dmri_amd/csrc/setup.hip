@@ -77,32 +77,31 @@ __global__ void gram_kernel(const double* __restrict__ A, int N, int K, double* 
     const int p = blockIdx.y, q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= K) return;
     double acc = 0.0;
+    // unrolled so that several loads are in flight (the loop is latency-bound)
+#pragma unroll 8
     for (int i = 0; i < N; ++i) acc += A[size_t(i) * K + p] * A[size_t(i) * K + q];
     G[size_t(p) * K + q] = acc;
 }
 
 
-// S_out = S_in S_in / max|S_in S_in| for a symmetric K x K matrix (one
-// thread per entry; the scale keeps repeated squaring in range).
-__global__ void square_scaled_kernel(const double* __restrict__ S, int K, double* __restrict__ out) {
+// S_out = S S / trace(S)^2 for a symmetric PSD K x K matrix S (one thread per
+// entry). With r = l_max / trace in [1/K, 1], the output's largest
+// eigenvalue is r^2 >= 1/K^2 and its own ratio is >= r, so repeated squaring
+// stays in range. Every block sums the trace in the same order (and the
+// butterfly leaves the same sum on every lane), so the output is symmetric
+// bit for bit. Launched with one wave (64 threads) per block.
+__global__ void __launch_bounds__(64) square_scaled_kernel(const double* __restrict__ S, int K,
+                                                           double* __restrict__ out) {
     const int p = blockIdx.y, q = blockIdx.x * blockDim.x + threadIdx.x;
+    double tr = 0.0;
+    for (int i = threadIdx.x; i < K; i += 64) tr += S[size_t(i) * K + i];
+    for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);  // the same sum on every lane
+    const double f = tr > 0.0 ? 1.0 / tr : 0.0;
     if (q >= K) return;
     double acc = 0.0;
+#pragma unroll 8
     for (int i = 0; i < K; ++i) acc += S[size_t(p) * K + i] * S[size_t(i) * K + q];
-    out[size_t(p) * K + q] = acc;
-}
-
-__global__ void __launch_bounds__(1024) normalize_max_kernel(double* __restrict__ S, int n) {
-    __shared__ double red[16];
-    double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(S[i]));
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    m = 0.0;
-    for (int i = 0; i < int(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
-    const double inv = m > 0.0 ? 1.0 / m : 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) S[i] *= inv;
+    out[size_t(p) * K + q] = (acc * f) * f;
 }
 
 __device__ double block_sum(double v, double* red) {
@@ -117,9 +116,9 @@ __device__ double block_sum(double v, double* red) {
 }
 
 // Largest eigenvalue of the symmetric PSD Gram matrix G. The dominant
-// eigenvector is found by power iteration on P = G^16 (scaled), whose
-// eigenvalue ratio is (l2/l1)^16: SHORE Gram matrices have l2/l1 ~ 0.86-0.94,
-// so P's ratio is <= 0.37 and a few dozen sweeps pin the vector to ~1e-12.
+// eigenvector is found by power iteration on P = G^64 (scaled), whose
+// eigenvalue ratio is (l2/l1)^64: SHORE Gram matrices have l2/l1 ~ 0.86-0.94,
+// so P's ratio is <= 0.02 and a handful of sweeps pin the vector to ~1e-13.
 // The eigenvalue is then the Rayleigh quotient v^T G v / v^T v with G itself,
 // polished by power sweeps on G until it stops moving (its error is quadratic
 // in the eigenvector error, so it reaches the FP64 floor; it never exceeds
@@ -137,8 +136,12 @@ __global__ void __launch_bounds__(1024) power_iteration_kernel(const double* __r
     for (int it = 0; it < 400; ++it) {
         double nrm = 0.0, diff = 0.0;
         for (int p = threadIdx.x; p < K; p += blockDim.x) {
+            // P is symmetric bit for bit (a product of symmetric matrices
+            // summed in the same order for (p, q) and (q, p)), so row p is
+            // read as column p: consecutive threads read consecutive entries
             double acc = 0.0;
-            for (int q = 0; q < K; ++q) acc += P[size_t(p) * K + q] * v[q];
+#pragma unroll 8
+            for (int q = 0; q < K; ++q) acc += P[size_t(q) * K + p] * v[q];
             w[p] = acc;
             nrm += acc * acc;
         }
@@ -167,7 +170,8 @@ __global__ void __launch_bounds__(1024) power_iteration_kernel(const double* __r
         double part = 0.0, nrm = 0.0, vpart = 0.0;
         for (int p = threadIdx.x; p < K; p += blockDim.x) {
             double acc = 0.0;
-            for (int q = 0; q < K; ++q) acc += G[size_t(p) * K + q] * v[q];
+#pragma unroll 8
+            for (int q = 0; q < K; ++q) acc += G[size_t(q) * K + p] * v[q];  // G symmetric, as P
             w[p] = acc;
             part += acc * v[p];
             nrm += acc * acc;
@@ -435,14 +439,13 @@ void launch_gram(const double* d_A, int N, int K, double* d_G, hipStream_t s) {
 }
 
 void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStream_t s) {
-    // P = G^16 by four scaled squarings (ping-pong buffers)
+    // P = G^64 (scaled) by six squarings (ping-pong buffers)
     DevBuf<double> p0(size_t(K) * K), p1(size_t(K) * K);
     const double* src = d_G;
     double* bufs[2] = {p0.p, p1.p};
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 6; ++i) {
         double* dst = bufs[i & 1];
         hipLaunchKernelGGL(square_scaled_kernel, dim3(blocks_for(K, 64), K), dim3(64), 0, s, src, K, dst);
-        hipLaunchKernelGGL(normalize_max_kernel, dim3(1), dim3(1024), 0, s, dst, K * K);
         src = dst;
     }
     hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(1024), size_t(2 * K) * sizeof(double), s, src, d_G, K,
