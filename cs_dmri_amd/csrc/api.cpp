@@ -1511,26 +1511,11 @@ int shorefit_microstructure(const float* coefficients, const uint8_t* mask, int6
         DevBuf<float> raw(size_t(6) * std::max<int64_t>(V, 1));
         launch_micro(c, vox.p, V, K, T, dirs, raw.p, s);
         // src/io/microstructure.rs:215-268, 303-326: threshold = factor x p99
-        // of the finite values (index round((n - 1) 0.99) of the sorted list)
-        MicroThresholds th{};
-        if (outlier_factor > 0.f && V > 0) {
-            std::vector<float> h(size_t(6) * V);
-            SF_HIP(hipMemcpyAsync(h.data(), raw.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, s));
-            SF_HIP(hipStreamSynchronize(s));
-            for (int q = 0; q < 6; ++q) {
-                std::vector<float> f;
-                f.reserve(V);
-                for (int64_t v = 0; v < V; ++v) {
-                    const float x = h[size_t(q) * V + v];
-                    if (std::isfinite(x)) f.push_back(x);
-                }
-                if (f.empty()) continue;
-                const size_t i = std::min(size_t(std::llround(0.99 * double(f.size() - 1))), f.size() - 1);
-                std::nth_element(f.begin(), f.begin() + i, f.end());
-                th.active[q] = 1;
-                th.thr[q] = outlier_factor * f[i];
-            }
-        }
+        // of the finite values (index round((n - 1) 0.99) of the sorted list),
+        // selected on the device
+        DevBuf<MicroThresholds> th(1);
+        SF_HIP(hipMemsetAsync(th.p, 0, sizeof(MicroThresholds), s));
+        if (outlier_factor > 0.f && V > 0) launch_micro_thresholds(raw.p, V, outlier_factor, th.p, s);
         float* user[6] = {rtop, rtap, rtpp, msd, qiv, ng};
         DevBuf<float> bo[6];
         float* maps[6];
@@ -1542,7 +1527,7 @@ int shorefit_microstructure(const float* coefficients, const uint8_t* mask, int6
             }
             if (maps[q] && nv) SF_HIP(hipMemsetAsync(maps[q], 0, sizeof(float) * nv, s));
         }
-        launch_micro_scatter(raw.p, vox.p, V, th, maps, s);
+        launch_micro_scatter(raw.p, vox.p, V, th.p, maps, s);
         if (!dev)
             for (int q = 0; q < 6; ++q)
                 if (user[q] && nv)

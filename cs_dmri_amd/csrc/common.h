@@ -334,7 +334,9 @@ void launch_apply_rows(const float* coef, const int64_t* vox, int64_t V, int K, 
                        float* out, hipStream_t s);
 void launch_micro(const float* coef, const int64_t* vox, int64_t V, int K, const MicroTables& T, const float* dirs,
                   float* raw, hipStream_t s);
-void launch_micro_scatter(const float* raw, const int64_t* vox, int64_t V, const MicroThresholds& th, float* const* maps,
-                          hipStream_t s);
+// Fit-failure thresholds of the six maps into d_th (device), all on the device.
+void launch_micro_thresholds(const float* raw, int64_t V, float factor, MicroThresholds* d_th, hipStream_t s);
+void launch_micro_scatter(const float* raw, const int64_t* vox, int64_t V, const MicroThresholds* d_th,
+                          float* const* maps, hipStream_t s);
 
 }  // namespace shorefit

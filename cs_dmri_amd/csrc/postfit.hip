@@ -98,45 +98,78 @@ double alpha_factor_h(unsigned n, unsigned ell, double zeta) {
 
 // out[v][r] = sum_k M[r][k] c_v[k] (FP64) for the voxels of the list (or all
 // voxels when vox is null). With skip_zero, an all-zero coefficient vector
-// gives zeros (src/synth.rs:57-61).
-template <int RB>
-__global__ void __launch_bounds__(256) apply_rows_kernel(const float* __restrict__ coef, const int64_t* __restrict__ vox,
-                                                         int64_t V, int K, const double* __restrict__ M, int R,
-                                                         int skip_zero, float* __restrict__ out) {
-    extern __shared__ float sc[];  // [K][65]
-    __shared__ int nz[64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t v0 = int64_t(blockIdx.x) * 64;
-    if (threadIdx.x < 64) nz[threadIdx.x] = 0;
+// gives zeros (src/synth.rs:57-61). Workgroup = 8 waves x 64 lanes x VPL
+// voxels per lane, their coefficients staged in LDS (coefficient-major);
+// wave w takes row blocks of RB rows and reads M transposed and padded
+// (MT[k][r], Rp >= R rows) with wave-uniform scalar loads, one RB-row
+// stretch per k, each element feeding VPL FMAs per lane. A workgroup whose
+// voxels are all zero (outside the mask) skips the products.
+template <int RB, int VPL>
+__global__ void __launch_bounds__(512) apply_rows_kernel(const float* __restrict__ coef, const int64_t* __restrict__ vox,
+                                                         int64_t V, int K, const double* __restrict__ MT, int Rp,
+                                                         int R, int skip_zero, float* __restrict__ out) {
+    constexpr int NV = 64 * VPL;
+    extern __shared__ float sc[];  // [K][NV + 1]
+    __shared__ int nz[NV];
+    __shared__ int any_nz;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t v0 = int64_t(blockIdx.x) * NV;
+    for (int t = threadIdx.x; t < NV; t += 512) nz[t] = 0;
+    if (threadIdx.x == 0) any_nz = 0;
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * K; t += 256) {
+    for (int t = threadIdx.x; t < NV * K; t += 512) {
         const int vj = t / K, k = t % K;
         const int64_t v = v0 + vj;
         float x = 0.f;
         if (v < V) x = coef[(vox ? vox[v] : v) * int64_t(K) + k];
-        sc[k * 65 + vj] = x;
-        if (x != 0.f) nz[vj] = 1;
+        sc[k * (NV + 1) + vj] = x;
+        if (x != 0.f) {
+            nz[vj] = 1;
+            any_nz = 1;
+        }
     }
     __syncthreads();
-    const int64_t v = v0 + lane;
-    const bool valid = v < V;
-    const int64_t idx = valid ? (vox ? vox[v] : v) : 0;
-    const bool zero = skip_zero && nz[lane] == 0;
-    for (int r0 = wave * RB; r0 < R; r0 += 4 * RB) {
-        double acc[RB];
+    const bool skip = skip_zero && any_nz == 0;
+    for (int r0 = wave * RB; r0 < R; r0 += 8 * RB) {
+        double acc[VPL][RB];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) acc[rb] = 0.0;
-        for (int k = 0; k < K; ++k) {
-            const double ck = double(sc[k * 65 + lane]);
+        for (int i = 0; i < VPL; ++i)
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-                if (r0 + rb < R) acc[rb] = fma(M[size_t(r0 + rb) * K + k], ck, acc[rb]);
+            for (int rb = 0; rb < RB; ++rb) acc[i][rb] = 0.0;
+        if (!skip) {
+            for (int k = 0; k < K; ++k) {
+                double ck[VPL];
+#pragma unroll
+                for (int i = 0; i < VPL; ++i) ck[i] = double(sc[k * (NV + 1) + lane + 64 * i]);
+                const double* mrow = MT + size_t(k) * Rp + r0;
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    const double m = mrow[rb];
+#pragma unroll
+                    for (int i = 0; i < VPL; ++i) acc[i][rb] = fma(m, ck[i], acc[i][rb]);
+                }
+            }
         }
-        if (valid)
+#pragma unroll
+        for (int i = 0; i < VPL; ++i) {
+            const int64_t v = v0 + lane + 64 * i;
+            if (v >= V) continue;
+            const int64_t idx = vox ? vox[v] : v;
+            const bool zero = skip_zero && nz[lane + 64 * i] == 0;
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
-                if (r0 + rb < R) out[idx * R + r0 + rb] = zero ? 0.f : float(acc[rb]);
+                if (r0 + rb < R) out[idx * R + r0 + rb] = zero ? 0.f : float(acc[i][rb]);
+        }
     }
+}
+
+// MT[k][r] = M[r][k] for r < R, zero for R <= r < Rp.
+__global__ void transpose_pad_kernel(const double* __restrict__ M, int R, int K, int Rp, double* __restrict__ MT) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(K) * Rp) return;
+    const int k = int(t / Rp), r = int(t % Rp);
+    MT[t] = r < R ? M[size_t(r) * K + k] : 0.0;
 }
 
 // Associated Legendre P_l^m(x) with the Condon-Shortley phase (src/math.rs:70-101).
@@ -256,16 +289,81 @@ __global__ void micro_kernel(const float* __restrict__ coef, const int64_t* __re
 
 // maps[q][vox[v]] = raw[q][v], NaN where finite and above thr[q].
 __global__ void micro_scatter_kernel(const float* __restrict__ raw, const int64_t* __restrict__ vox, int64_t V,
-                                     MicroThresholds th, float* m0, float* m1, float* m2, float* m3, float* m4,
-                                     float* m5) {
+                                     const MicroThresholds* __restrict__ th, float* m0, float* m1, float* m2,
+                                     float* m3, float* m4, float* m5) {
     const int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const int64_t idx = vox ? vox[v] : v;
     float* maps[6] = {m0, m1, m2, m3, m4, m5};
     for (int q = 0; q < 6; ++q) {
         float x = raw[q * V + v];
-        if (th.active[q] && isfinite(x) && x > th.thr[q]) x = __builtin_nanf("");
+        if (th->active[q] && isfinite(x) && x > th->thr[q]) x = __builtin_nanf("");
         if (maps[q]) maps[q][idx] = x;
+    }
+}
+
+// Order-preserving map of a float's bits to an unsigned key, and back.
+__device__ __forceinline__ uint32_t float_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+}
+
+// Fit-failure thresholds (src/io/microstructure.rs:215-268, 303-326): for
+// map q (one workgroup each), factor x the value of rank round((n - 1) 0.99)
+// among the n finite values in ascending order, found by a four-pass radix
+// selection over 8-bit digits of the order-preserving keys (the same element
+// a full sort, or nth_element, puts at that rank). No finite value: inactive.
+__global__ void __launch_bounds__(1024) micro_threshold_kernel(const float* __restrict__ raw, int64_t V, float factor,
+                                                               MicroThresholds* __restrict__ th) {
+    const int q = blockIdx.x;
+    const float* x = raw + q * V;
+    __shared__ unsigned long long cnt;
+    __shared__ unsigned hist[256];
+    __shared__ uint32_t prefix_s, mask_s;
+    __shared__ long long rank_s;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    unsigned long long c = 0;
+    for (int64_t v = threadIdx.x; v < V; v += blockDim.x) c += isfinite(x[v]) ? 1 : 0;
+    atomicAdd(&cnt, c);
+    __syncthreads();
+    const long long n = (long long)cnt;
+    if (n == 0) {
+        if (threadIdx.x == 0) th->active[q] = 0;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        rank_s = std::min((long long)llround(0.99 * double(n - 1)), n - 1);
+        prefix_s = 0u;
+        mask_s = 0u;
+    }
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0u;
+        __syncthreads();
+        const uint32_t prefix = prefix_s, mask = mask_s;
+        for (int64_t v = threadIdx.x; v < V; v += blockDim.x) {
+            const float f = x[v];
+            if (!isfinite(f)) continue;
+            const uint32_t k = float_key(f);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long r = rank_s;
+            int d = 0;
+            while (d < 255 && r >= (long long)hist[d]) r -= hist[d++];
+            rank_s = r;
+            prefix_s = prefix | (uint32_t(d) << shift);
+            mask_s = mask | (255u << shift);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        th->active[q] = 1;
+        th->thr[q] = factor * key_float(prefix_s);
     }
 }
 
@@ -348,14 +446,23 @@ void micro_tables_host(unsigned R, double zeta, int units, MicroTables& T) {
 void launch_apply_rows(const float* coef, const int64_t* vox, int64_t V, int K, const double* M, int R, bool skip_zero,
                        float* out, hipStream_t s) {
     if (V <= 0) return;
-    const size_t lds = size_t(K) * 65 * sizeof(float);
-    auto kern = apply_rows_kernel<8>;
+    constexpr int VPL = 2;
+    const bool small = R <= 64;  // 8-row blocks keep more waves busy on short matrices
+    const int RB = small ? 8 : 16;
+    const int Rp = (R + RB - 1) / RB * RB;
+    DevBuf<double> MT(size_t(K) * Rp);
+    hipLaunchKernelGGL(transpose_pad_kernel, dim3(unsigned((int64_t(K) * Rp + 255) / 256)), dim3(256), 0, s, M, R, K,
+                       Rp, MT.p);
+    SF_HIP(hipGetLastError());
+    const size_t lds = size_t(K) * (64 * VPL + 1) * sizeof(float);
+    auto kern = small ? apply_rows_kernel<8, VPL> : apply_rows_kernel<16, VPL>;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int(lds));
-    hipLaunchKernelGGL(kern, dim3(unsigned((V + 63) / 64)), dim3(256), lds, s, coef, vox, V, K, M, R,
-                       int(skip_zero), out);
+    hipLaunchKernelGGL(kern, dim3(unsigned((V + 64 * VPL - 1) / (64 * VPL))), dim3(512), lds, s, coef, vox, V, K, MT.p,
+                       Rp, R, int(skip_zero), out);
     SF_HIP(hipGetLastError());
+    sync_if_unscoped(s);  // MT is released at scope exit
 }
 
 void launch_micro(const float* coef, const int64_t* vox, int64_t V, int K, const MicroTables& T, const float* dirs,
@@ -365,11 +472,17 @@ void launch_micro(const float* coef, const int64_t* vox, int64_t V, int K, const
     SF_HIP(hipGetLastError());
 }
 
-void launch_micro_scatter(const float* raw, const int64_t* vox, int64_t V, const MicroThresholds& th, float* const* maps,
-                          hipStream_t s) {
+void launch_micro_thresholds(const float* raw, int64_t V, float factor, MicroThresholds* d_th, hipStream_t s) {
     if (V <= 0) return;
-    hipLaunchKernelGGL(micro_scatter_kernel, dim3(unsigned((V + 255) / 256)), dim3(256), 0, s, raw, vox, V, th, maps[0],
-                       maps[1], maps[2], maps[3], maps[4], maps[5]);
+    hipLaunchKernelGGL(micro_threshold_kernel, dim3(6), dim3(1024), 0, s, raw, V, factor, d_th);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_micro_scatter(const float* raw, const int64_t* vox, int64_t V, const MicroThresholds* d_th,
+                          float* const* maps, hipStream_t s) {
+    if (V <= 0) return;
+    hipLaunchKernelGGL(micro_scatter_kernel, dim3(unsigned((V + 255) / 256)), dim3(256), 0, s, raw, vox, V, d_th,
+                       maps[0], maps[1], maps[2], maps[3], maps[4], maps[5]);
     SF_HIP(hipGetLastError());
 }
 
