@@ -226,6 +226,43 @@ __global__ void __launch_bounds__(256) cholesky_kernel(double* __restrict__ G, i
     if (threadIdx.x == 0) status[0] = bad;
 }
 
+// The same right-looking factorization (the same operations in the same
+// order, so the same bits) on the packed lower triangle held in LDS
+// (L[a][b] at a (a + 1) / 2 + b): the trailing updates are LDS traffic
+// instead of L2 round trips. Used when K (K + 1) / 2 doubles fit in LDS.
+__global__ void __launch_bounds__(1024) cholesky_lds_kernel(double* __restrict__ G, int K, int* __restrict__ status) {
+    extern __shared__ double Lp[];
+    __shared__ int bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) bad = 0;
+    for (int a = 0; a < K; ++a)
+        for (int b = tid; b <= a; b += nt) Lp[a * (a + 1) / 2 + b] = G[size_t(a) * K + b];
+    __syncthreads();
+    for (int j = 0; j < K; ++j) {
+        const double d = Lp[j * (j + 1) / 2 + j];
+        if (!(d > 0.0)) {
+            if (tid == 0) bad = j + 1;
+            break;
+        }
+        const double s = sqrt(d);
+        __syncthreads();  // every thread has read the pivot
+        for (int i = j + 1 + tid; i < K; i += nt) Lp[i * (i + 1) / 2 + j] /= s;
+        if (tid == 0) Lp[j * (j + 1) / 2 + j] = s;
+        __syncthreads();
+        // trailing update: wave w takes rows j + 1 + w + 16 i, lanes the columns
+        const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+        for (int a = j + 1 + wave; a < K; a += nw) {
+            const double laj = Lp[a * (a + 1) / 2 + j];
+            for (int b = j + 1 + lane; b <= a; b += 64) Lp[a * (a + 1) / 2 + b] -= laj * Lp[b * (b + 1) / 2 + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int a = 0; a < K; ++a)
+        for (int b = tid; b < K; b += nt) G[size_t(a) * K + b] = b > a ? 0.0 : Lp[a * (a + 1) / 2 + b];
+    if (tid == 0) status[0] = bad;
+}
+
 // Fragment-major operand of __builtin_amdgcn_mfma_f32_16x16x4f32: for
 // 16x16 block (rb, cb) and lane l, four floats X[16rb + l%16][16cb + 4(l/16) + r].
 __global__ void pack_rowblock_kernel(const double* __restrict__ src, int rows, int cols, int ld, int transpose,
@@ -290,6 +327,44 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
 // Column n of P = (L L^T)^-1 A^T is the solution for right-hand side row n of
 // A (forward then backward substitution), computed in place in P's column
 // (P is K x N row-major, so threads of consecutive n touch consecutive words).
+// P = (L L^T)^-1 A^T for CB columns per workgroup: L (packed lower triangle)
+// and the workgroup's CB columns in LDS, 16 threads per column summing
+// strided partial dot products, combined by a butterfly.
+template <int CB>
+__global__ void __launch_bounds__(CB * 16) chol_solve_lds_kernel(const double* __restrict__ Lf, int K,
+                                                                 const double* __restrict__ A, int N,
+                                                                 double* __restrict__ P) {
+    extern __shared__ double sm[];
+    double* Lp = sm;                              // K (K + 1) / 2
+    double* Y = sm + size_t(K) * (K + 1) / 2;     // [K][CB]
+    const int tid = threadIdx.x, nt = CB * 16;
+    const int c = tid >> 4, u = tid & 15;
+    const int n0 = blockIdx.x * CB, n = n0 + c;
+    for (int a = 0; a < K; ++a)
+        for (int b = tid; b <= a; b += nt) Lp[a * (a + 1) / 2 + b] = Lf[size_t(a) * K + b];
+    for (int t = tid; t < K * CB; t += nt) {
+        const int i = t / CB, cc = t % CB;
+        Y[t] = n0 + cc < N ? A[size_t(n0 + cc) * K + i] : 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < K; ++i) {  // L y = a
+        double part = 0.0;
+        for (int k = u; k < i; k += 16) part += Lp[i * (i + 1) / 2 + k] * Y[k * CB + c];
+        for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
+        if (u == 0) Y[i * CB + c] = (Y[i * CB + c] - part) / Lp[i * (i + 1) / 2 + i];
+        __syncthreads();
+    }
+    for (int i = K - 1; i >= 0; --i) {  // L^T p = y
+        double part = 0.0;
+        for (int k = i + 1 + u; k < K; k += 16) part += Lp[k * (k + 1) / 2 + i] * Y[k * CB + c];
+        for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
+        if (u == 0) Y[i * CB + c] = (Y[i * CB + c] - part) / Lp[i * (i + 1) / 2 + i];
+        __syncthreads();
+    }
+    if (n < N)
+        for (int i = u; i < K; i += 16) P[size_t(i) * N + n] = Y[i * CB + c];
+}
+
 __global__ void chol_solve_columns_kernel(const double* __restrict__ Lf, int K, const double* __restrict__ A, int N,
                                           double* __restrict__ P) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -463,8 +538,17 @@ void sync_if_unscoped(hipStream_t s) {
     if (!stream_scope_active()) SF_HIP(hipStreamSynchronize(s));
 }
 
+constexpr size_t kCholLds = 150 * 1024;  // LDS the packed-triangle kernels may take
+
 void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s) {
-    hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(256), 0, s, d_G, K, d_status);
+    const size_t lds = size_t(K) * (K + 1) / 2 * sizeof(double);
+    if (lds <= kCholLds) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(cholesky_lds_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL(cholesky_lds_kernel, dim3(1), dim3(1024), lds, s, d_G, K, d_status);
+    } else {
+        hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(256), 0, s, d_G, K, d_status);
+    }
     SF_HIP(hipGetLastError());
 }
 
@@ -491,7 +575,16 @@ void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, u
 }
 
 void launch_chol_solve_columns(const double* d_L, int K, const double* d_A, int N, double* d_P, hipStream_t s) {
-    hipLaunchKernelGGL(chol_solve_columns_kernel, dim3(blocks_for(N, 64)), dim3(64), 0, s, d_L, K, d_A, N, d_P);
+    constexpr int CB = 16;  // columns per workgroup
+    const size_t lds = (size_t(K) * (K + 1) / 2 + size_t(K) * CB) * sizeof(double);
+    if (lds <= kCholLds) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(chol_solve_lds_kernel<CB>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL(chol_solve_lds_kernel<CB>, dim3(unsigned((N + CB - 1) / CB)), dim3(CB * 16), lds, s, d_L,
+                           K, d_A, N, d_P);
+    } else {
+        hipLaunchKernelGGL(chol_solve_columns_kernel, dim3(blocks_for(N, 64)), dim3(64), 0, s, d_L, K, d_A, N, d_P);
+    }
     SF_HIP(hipGetLastError());
 }
 
