@@ -68,8 +68,9 @@ def parse():
     p.add_argument("--max-iter", type=int, default=500)
     p.add_argument("--alpha-mode", default="fixed", choices=["fixed", "alpha-ratio", "path-bic", "l2-anchored"],
                    help="alpha strategy (the headline line is the fixed-alpha fit)")
-    p.add_argument("--workload", default="shore8", choices=["shore8", "graddev", "adjoint"],
-                   help="shore8: the shared-design whole-brain fit (the headline line); graddev: the same volume "
+    p.add_argument("--workload", default="shore8", choices=["shore8", "tikhonov", "graddev", "adjoint"],
+                   help="shore8: the shared-design whole-brain fit (the headline line); tikhonov: the same volume "
+                        "fitted with Tikhonov L2 (lambda_N = lambda_L = 1e-2); graddev: the same volume "
                         "with per-voxel gradient-deviation tensors and mixed SNR (one line per --orders entry)")
     p.add_argument("--orders", default="6,8,10", help="radial orders of the graddev workload")
     p.add_argument("--tune", action="store_true", help="graddev: autotune the tile shapes first")
@@ -147,6 +148,97 @@ def postfit_timings(L, d_coef, d_mask, wl, K, N, dev):
         ms = (time.perf_counter() - t0) / 3 * 1e3
         out[name] = {"ms": round(ms, 3), "algorithmic_bytes": nbytes, "GB_s": round(nbytes / ms / 1e6, 1)}
     return out
+
+
+def tikhonov_bench(args):
+    """The whole-brain volume (250,176 voxels, order 8) fitted with the
+    Tikhonov L2 solver, lambda_N = lambda_L = 1e-2 (src/solver/tikhonov.rs:26-76).
+    A step is one whole ``shorefit_fit``: design, G, Cholesky of G + Lambda,
+    P = (G + Lambda)^-1 A^T (FP64, on the device), then c = P s and the
+    residual per voxel on FP32 MFMA (``tikhonov_kernel``). The roofline is that
+    kernel's: 4 N K algorithmic flops per voxel (P s and A c) against the FP32
+    MFMA peak; its compulsory bytes, 4 (N + K + 1) per voxel, are reported
+    beside it."""
+    import torch
+
+    from cs_dmri_amd import _lib as L
+    from cs_dmri_amd import workloads as W
+    from cs_dmri_amd.models import _spec
+
+    dev = torch.device("cuda:0")
+    wl = W.config2()
+    g = wl.gtab
+    K = L.lib().shorefit_n_coeffs(wl.radial_order)
+    N = len(g.bvals)
+    V = wl.n_voxels
+    shape = wl.mask.shape
+    d_data = torch.from_numpy(wl.data).to(dev)
+    d_mask = torch.from_numpy(wl.mask.astype(np.uint8)).to(dev)
+    d_coef = torch.empty(shape + (K,), dtype=torch.float32, device=dev)
+    d_res = torch.empty(shape, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    spec = _spec(wl.radial_order, wl.zeta, "l2", "fixed", 1e-3, 1e-3, 20, None, 0.05, args.max_iter, 1e-6,
+                 False, 1e-2, 1e-2, True, False, "fp32", args.refresh)
+    grads, keep = L.gradients(g.bvals, g.bvecs, g.big_delta, g.small_delta, g.b0_threshold)
+    vol = L.ShorefitVolume(d_data.data_ptr(), d_mask.data_ptr(), shape[0], shape[1], shape[2], 1, 0)
+
+    def step():
+        out = L.ShorefitOutput()
+        out.coefficients = d_coef.data_ptr()
+        out.residual_l2 = d_res.data_ptr()
+        L.check(L.lib().shorefit_fit(C.byref(grads), C.byref(vol), C.byref(spec), C.byref(out), None))
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = []
+    for _ in range(args.steps):
+        kernel_ms.append(step().fit_ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_s = float(np.mean(kernel_ms)) * 1e-3
+    flops = 4.0 * N * K * V
+    nbytes = 4.0 * (N + K + 1) * V
+    result = {
+        "metric": "voxel fits/s (SHORE-8 Tikhonov L2)", "value": round(V * args.steps / wall, 1),
+        "unit": "voxel-fits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded multi-tensor + Rician phantom)",
+        "config": {"workload": "shore8-cartesian258-96x96x60-ellipsoid", "voxels": V, "n_grads": N, "n_coeffs": K,
+                   "radial_order": 8, "solver": "tikhonov-l2 lambda_n=lambda_l=1e-2"},
+        "roofline": {"bound": "mfma", "achieved": round(flops / kern_s / 1e12, 3), "peak": PEAK["fp32"],
+                     "unit": "TFLOP/s", "frac": round(flops / kern_s / 1e12 / PEAK["fp32"], 4), "traffic": None,
+                     "kernel": "tikhonov_kernel<10>", "kernel_ms": round(kern_s * 1e3, 3),
+                     "flops_per_launch": flops, "algorithmic_bytes": nbytes,
+                     "GB_s": round(nbytes / kern_s / 1e9, 1)},
+    }
+    if not args.no_cpu:
+        from oracle import load
+        orc = load()
+        A = orc.design_matrix(wl.radial_order, wl.zeta, g.bvals, g.bvecs, g.big_delta, g.small_delta)
+        dn, dl = orc.regularization(wl.radial_order)
+        S = wl.data[wl.mask]
+        pick = np.arange(V)  # a Tikhonov fit is one solve per voxel: the whole mask takes about a second
+        threads = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        ref = orc.fit_volume(S[pick].reshape(len(pick), 1, 1, N), None, A, regularization="l2", dn=dn, dl=dl,
+                             lambda_n=1e-2, lambda_l=1e-2, n_threads=threads)
+        cpu_wall = time.perf_counter() - t0
+        cr = ref["coefficients"].reshape(len(pick), K).astype(np.float64)
+        cg = d_coef.cpu().numpy()[wl.mask][pick].astype(np.float64)
+        err = np.linalg.norm(cg - cr, axis=1) / np.maximum(np.linalg.norm(cr, axis=1), 1e-30)
+        rg = d_res.cpu().numpy()[wl.mask][pick].astype(np.float64)
+        rr = ref["residual_l2"].reshape(len(pick)).astype(np.float64)
+        result["cpu_baseline"] = {"value": round(len(pick) / cpu_wall, 2), "unit": "voxel-fits/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"all {len(pick)} masked voxels, same fit settings, "
+                                            f"FP64 oracle (C++), {cpu_wall:.2f} s wall"}
+        result["accuracy_vs_fp64"] = {"sample": len(pick), "coef_rel_err_median": float(np.median(err)),
+                                      "coef_rel_err_max": float(err.max()),
+                                      "residual_rel_err_max": float(np.max(np.abs(rg - rr) / np.maximum(rr, 1e-30)))}
+    print(json.dumps(result), flush=True)
 
 
 def graddev_bench(args):
@@ -425,6 +517,9 @@ def main():
         return
     if args.precision_report:
         precision_report()
+        return
+    if args.workload == "tikhonov":
+        tikhonov_bench(args)
         return
     if args.workload == "graddev":
         graddev_bench(args)

@@ -397,17 +397,17 @@ void launch_chol_solve_columns(const double* d_L, int K, const double* d_A, int 
 namespace {
 void pack_tikhonov(const Design& d, const double* dn, const double* dl, double lam_n, double lam_l, TikOps& ops,
                    hipStream_t s) {
-    // src/solver/tikhonov.rs:33-44
-    std::vector<double> G(size_t(d.K) * d.K);
-    SF_HIP(hipMemcpyAsync(G.data(), d.G.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, s));
-    SF_HIP(hipStreamSynchronize(s));
+    // src/solver/tikhonov.rs:33-44: G + lambda_N D_N + lambda_L D_L, on the device
+    std::vector<double> diag(size_t(2) * d.K, 0.0);
     for (int i = 0; i < d.K; ++i) {
-        if (dn) G[size_t(i) * d.K + i] += lam_n * dn[i];
-        if (dl) G[size_t(i) * d.K + i] += lam_l * dl[i];
+        if (dn) diag[i] = dn[i];
+        if (dl) diag[d.K + i] = dl[i];
     }
-    DevBuf<double> Lf(G.size());
+    DevBuf<double> ddiag(diag.size());
+    SF_HIP(hipMemcpyAsync(ddiag.p, diag.data(), sizeof(double) * diag.size(), hipMemcpyHostToDevice, s));
+    DevBuf<double> Lf(size_t(d.K) * d.K);
     DevBuf<int> status(1);
-    SF_HIP(hipMemcpyAsync(Lf.p, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice, s));
+    launch_add_diagonals(d.G.p, d.K, dn ? ddiag.p : nullptr, lam_n, dl ? ddiag.p + d.K : nullptr, lam_l, Lf.p, s);
     launch_cholesky(Lf.p, d.K, status.p, s);
     int st = 0;
     SF_HIP(hipMemcpyAsync(&st, status.p, sizeof(int), hipMemcpyDeviceToHost, s));

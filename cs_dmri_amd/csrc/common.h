@@ -99,6 +99,10 @@ void launch_gram(const double* d_A, int N, int K, double* d_G, hipStream_t s);
 // Largest eigenvalue of the symmetric K x K matrix G by power iteration
 // (Rayleigh quotient), written to d_lambda[0].
 void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStream_t s);
+// out = G + lam_n diag(dn) + lam_l diag(dl) (K x K; dn / dl may be null),
+// the two diagonal terms added in that order.
+void launch_add_diagonals(const double* d_G, int K, const double* d_dn, double lam_n, const double* d_dl,
+                          double lam_l, double* d_out, hipStream_t s);
 // From the largest eigenvalue lam of A^T A: out[0] = L = max(lam / N, eps)
 // (src/solver/fista.rs:51,58), out[1] = c = (1 / L) / N, on the device.
 void launch_fit_scalars(const double* d_lam, int N, double* d_out, hipStream_t s);

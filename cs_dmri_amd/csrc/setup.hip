@@ -263,6 +263,19 @@ __global__ void __launch_bounds__(1024) cholesky_lds_kernel(double* __restrict__
     if (tid == 0) status[0] = bad;
 }
 
+__global__ void add_diagonals_kernel(const double* __restrict__ G, int K, const double* __restrict__ dn, double lam_n,
+                                     const double* __restrict__ dl, double lam_l, double* __restrict__ out) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(K) * K) return;
+    double g = G[t];
+    const int i = int(t / K);
+    if (t == int64_t(i) * K + i) {
+        if (dn) g += lam_n * dn[i];
+        if (dl) g += lam_l * dl[i];
+    }
+    out[t] = g;
+}
+
 // Fragment-major operand of __builtin_amdgcn_mfma_f32_16x16x4f32: for
 // 16x16 block (rb, cb) and lane l, four floats X[16rb + l%16][16cb + 4(l/16) + r].
 __global__ void pack_rowblock_kernel(const double* __restrict__ src, int rows, int cols, int ld, int transpose,
@@ -536,6 +549,13 @@ void launch_fit_scalars(const double* d_lam, int N, double* d_out, hipStream_t s
 
 void sync_if_unscoped(hipStream_t s) {
     if (!stream_scope_active()) SF_HIP(hipStreamSynchronize(s));
+}
+
+void launch_add_diagonals(const double* d_G, int K, const double* d_dn, double lam_n, const double* d_dl,
+                          double lam_l, double* d_out, hipStream_t s) {
+    hipLaunchKernelGGL(add_diagonals_kernel, dim3(blocks_for(int64_t(K) * K, 256)), dim3(256), 0, s, d_G, K, d_dn,
+                       lam_n, d_dl, lam_l, d_out);
+    SF_HIP(hipGetLastError());
 }
 
 constexpr size_t kCholLds = 150 * 1024;  // LDS the packed-triangle kernels may take
