@@ -68,9 +68,10 @@ def parse():
     p.add_argument("--max-iter", type=int, default=500)
     p.add_argument("--alpha-mode", default="fixed", choices=["fixed", "alpha-ratio", "path-bic", "l2-anchored"],
                    help="alpha strategy (the headline line is the fixed-alpha fit)")
-    p.add_argument("--workload", default="shore8", choices=["shore8", "tikhonov", "graddev", "adjoint"],
+    p.add_argument("--workload", default="shore8", choices=["shore8", "tikhonov", "graddev", "adjoint", "io"],
                    help="shore8: the shared-design whole-brain fit (the headline line); tikhonov: the same volume "
-                        "fitted with Tikhonov L2 (lambda_N = lambda_L = 1e-2); graddev: the same volume "
+                        "fitted with Tikhonov L2 (lambda_N = lambda_L = 1e-2); io: the volume's NIfTI read and "
+                        "coefficient write path; graddev: the same volume "
                         "with per-voxel gradient-deviation tensors and mixed SNR (one line per --orders entry)")
     p.add_argument("--orders", default="6,8,10", help="radial orders of the graddev workload")
     p.add_argument("--tune", action="store_true", help="graddev: autotune the tile shapes first")
@@ -148,6 +149,97 @@ def postfit_timings(L, d_coef, d_mask, wl, K, N, dev):
         ms = (time.perf_counter() - t0) / 3 * 1e3
         out[name] = {"ms": round(ms, 3), "algorithmic_bytes": nbytes, "GB_s": round(nbytes / ms / 1e6, 1)}
     return out
+
+
+def io_bench(args):
+    """The on-disk path around the fit at whole-brain size: the 96x96x60x258
+    series stored as int16 with scl_slope / scl_inter (the usual scanner
+    export) and as float32, read into the voxel-major f32 volume the fit
+    uses, and the (X, Y, Z, 145) coefficient volume written back. The device
+    kernel (``to_voxel_major_kernel``, a type conversion and a 4-axis
+    reversal) is HBM-bound: raw bytes in + 4 bytes out per element; its time
+    is measured with HIP events on its stream. Wall times of the whole calls
+    (disk, PCIe, kernel) are reported beside it; the CPU leg is numpy doing
+    the same conversion (astype + scale + transpose) on the host."""
+    import tempfile
+
+    import torch
+
+    from cs_dmri_amd import io as sio
+    from cs_dmri_amd import nifti as nf
+    from cs_dmri_amd import workloads as W
+    from cs_dmri_amd import _lib as L
+
+    wl = W.config2()
+    data = wl.data  # (96, 96, 60, 258) float32
+    nx, ny, nz, nt = data.shape
+    slope, inter = 0.05, 0.0
+    q = np.clip(np.rint(data / slope), -32768, 32767).astype(np.int16).transpose(3, 2, 1, 0)
+    dev = torch.device("cuda:0")
+    res = {"metric": "NIfTI DWI read to voxel-major f32 (GB/s of the device conversion)", "unit": "GB/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int16->f32", "data": "synthetic (the config-2 phantom, quantised)",
+           "config": {"workload": "dwi-96x96x60x258-int16-scl", "elements": int(data.size)}}
+    with tempfile.TemporaryDirectory() as td:
+        p16 = os.path.join(td, "dwi16.nii")
+        nf.save(p16, nf.new_header(data.shape, np.diag([2.0, 2.0, 2.0, 1.0]), datatype=4, bitpix=16), q)
+        h = nf.read_header(p16)
+        h["scl_slope"], h["scl_inter"] = slope, inter
+        nf.save(p16, h, q)
+        img = nf.load(p16)
+        raw = np.require(img.data.reshape(-1), requirements=["C", "W"])
+        d_raw = torch.from_numpy(raw.view(np.uint8)).to(dev)
+        out = torch.empty((nx, ny, nz, nt), dtype=torch.float32, device=dev)
+        stream = torch.cuda.current_stream(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for _ in range(args.warmup):
+            L.check(L.lib().shorefit_volume_to_voxel_major(d_raw.data_ptr(), 4, nx, ny, nz, nt, slope, inter,
+                                                           out.data_ptr(), 1, stream.cuda_stream))
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(args.steps):
+            e0.record(stream)
+            L.check(L.lib().shorefit_volume_to_voxel_major(d_raw.data_ptr(), 4, nx, ny, nz, nt, slope, inter,
+                                                           out.data_ptr(), 1, stream.cuda_stream))
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        k_ms = float(np.mean(ms))
+        nbytes = data.size * (2 + 4)
+        res["value"] = round(nbytes / k_ms / 1e6, 1)
+        res["ms_per_step"] = round(k_ms, 4)
+        res["roofline"] = {"bound": "hbm", "achieved": res["value"], "peak": 8000.0, "unit": "GB/s",
+                           "frac": round(nbytes / k_ms / 1e6 / 8000.0, 4), "traffic": None,
+                           "kernel": "to_voxel_major_kernel<int16_t>", "kernel_ms": round(k_ms, 4),
+                           "algorithmic_bytes": nbytes}
+        want = (q.astype(np.float32) * np.float32(slope) + np.float32(inter)).transpose(3, 2, 1, 0)
+        res["parity"] = {"bit_identical_to_numpy": bool(np.array_equal(out.cpu().numpy(), want))}
+        # whole calls: file -> device volume, host volume, coefficient file write
+        t0 = time.perf_counter()
+        v_dev = sio.read_dwi_volume(p16, device=dev)
+        torch.cuda.synchronize()
+        t_read_dev = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sio.read_dwi_volume(p16)
+        t_read_host = time.perf_counter() - t0
+        coef = torch.zeros((nx, ny, nz, 145), dtype=torch.float32, device=dev)
+        coef[..., :] = v_dev[..., :145]
+        side = {"n_coefficients": 145}
+        t0 = time.perf_counter()
+        sio.write_coefficients(os.path.join(td, "coef.nii"), coef, p16, side)
+        t_write = time.perf_counter() - t0
+        res["wall_s"] = {"read_to_device": round(t_read_dev, 3), "read_to_host": round(t_read_host, 3),
+                         "write_coefficients_from_device": round(t_write, 3)}
+        if not args.no_cpu:
+            t0 = time.perf_counter()
+            ref = np.ascontiguousarray((img.data.astype(np.float32) * np.float32(slope) + np.float32(inter))
+                                       .transpose(3, 2, 1, 0))
+            cpu_s = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": round(nbytes / cpu_s / 1e9, 2), "unit": "GB/s", "cores": 1,
+                                   "kind": "port", "sample": f"the whole series, numpy astype + scale + transpose, "
+                                                             f"{cpu_s:.2f} s"}
+            res["parity"]["cpu_equal"] = bool(np.array_equal(ref, want))
+    print(json.dumps(res), flush=True)
 
 
 def tikhonov_bench(args):
@@ -520,6 +612,9 @@ def main():
         return
     if args.workload == "tikhonov":
         tikhonov_bench(args)
+        return
+    if args.workload == "io":
+        io_bench(args)
         return
     if args.workload == "graddev":
         graddev_bench(args)

@@ -27,7 +27,7 @@ EXPORTS = (
     "shorefit_tournier_matrix", "shorefit_odf_sh", "shorefit_predict", "shorefit_microstructure",
     "shorefit_fit_graddev", "shorefit_graddev_variants", "shorefit_graddev_tune", "shorefit_graddev_gram",
     "shorefit_design_create", "shorefit_design_destroy", "shorefit_design_info", "shorefit_lasso_forward",
-    "shorefit_lasso_adjoint",
+    "shorefit_lasso_adjoint", "shorefit_volume_to_voxel_major", "shorefit_volume_to_file_order",
 )
 
 
@@ -86,7 +86,25 @@ _pd = C.POINTER(C.c_double)
 _pf = C.POINTER(C.c_float)
 
 
+def _torch_runtime_first() -> None:
+    """Starts torch's HIP runtime, when torch and a GPU are present, before
+    this library's. torch ships its own libamdhip64 beside the one the
+    library links; a process whose first GPU use is the library's runtime
+    then cannot start torch's ("No HIP GPUs are available"), while the other
+    order works. Loading the library therefore starts torch's first."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except RuntimeError:
+        pass
+
+
 def _load() -> C.CDLL:
+    _torch_runtime_first()
     if not LIB_PATH.exists():
         raise ImportError(
             f"{LIB_PATH} is not built; run `python -m cs_dmri_amd.build` (hipcc, gfx950). "
@@ -146,6 +164,12 @@ def _load() -> C.CDLL:
     lib.shorefit_lasso_forward.restype = C.c_int
     lib.shorefit_lasso_adjoint.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp]
     lib.shorefit_lasso_adjoint.restype = C.c_int
+    lib.shorefit_volume_to_voxel_major.argtypes = [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                                   C.c_float, C.c_float, _vp, C.c_int32, _vp]
+    lib.shorefit_volume_to_voxel_major.restype = C.c_int
+    lib.shorefit_volume_to_file_order.argtypes = [_vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, _vp, C.c_int32,
+                                                  _vp]
+    lib.shorefit_volume_to_file_order.restype = C.c_int
     return lib
 
 

@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libshorefit.so"
-SOURCES = ["setup.hip", "fista.hip", "postfit.hip", "graddev.hip", "adjoint.hip", "api.cpp"]
+SOURCES = ["setup.hip", "fista.hip", "postfit.hip", "graddev.hip", "adjoint.hip", "volume.hip", "api.cpp"]
 ARCH = "gfx950"
 
 

@@ -1484,6 +1484,72 @@ int shorefit_predict(const float* coefficients, int64_t nx, int64_t ny, int64_t 
     });
 }
 
+namespace {
+size_t nifti_element_size(int32_t datatype) {
+    switch (datatype) {
+        case 2: case 256: return 1;
+        case 4: case 512: return 2;
+        case 8: case 16: case 768: return 4;
+        case 64: return 8;
+        default:
+            throw Error{SHOREFIT_E_UNSUPPORTED, "NIfTI datatype " + std::to_string(datatype) +
+                                                    " is not supported (uint8, int8, int16, uint16, int32, uint32, "
+                                                    "float32, float64 are)"};
+    }
+}
+}  // namespace
+
+int shorefit_volume_to_voxel_major(const void* raw, int32_t datatype, int64_t nx, int64_t ny, int64_t nz, int64_t nt,
+                                   float scl_slope, float scl_inter, float* out, int32_t on_device, void* stream) {
+    return guarded([&] {
+        if (!raw || !out) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        check_dims(nx, ny, nz);
+        if (nt < 0) throw Error{SHOREFIT_E_INVALID, "negative volume dimension"};
+        const size_t es = nifti_element_size(datatype);
+        const size_t n = size_t(nx) * ny * nz * nt;
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        const bool dev = on_device != 0;
+        DevBuf<uint8_t> braw;
+        DevBuf<float> bout;
+        const void* r = raw;
+        float* o = out;
+        if (!dev) {
+            braw.alloc(n * es);
+            if (n) SF_HIP(hipMemcpyAsync(braw.p, raw, n * es, hipMemcpyHostToDevice, s));
+            r = braw.p;
+            bout.alloc(n);
+            o = bout.p;
+        }
+        launch_to_voxel_major(r, datatype, nx, ny, nz, nt, scl_slope, scl_inter, o, s);
+        if (!dev && n) SF_HIP(hipMemcpyAsync(out, o, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int shorefit_volume_to_file_order(const float* in, int64_t nx, int64_t ny, int64_t nz, int64_t nt, float* out,
+                                  int32_t on_device, void* stream) {
+    return guarded([&] {
+        if (!in || !out) throw Error{SHOREFIT_E_INVALID, "null argument"};
+        check_dims(nx, ny, nz);
+        if (nt < 0) throw Error{SHOREFIT_E_INVALID, "negative volume dimension"};
+        const size_t n = size_t(nx) * ny * nz * nt;
+        hipStream_t s = as_stream(stream);
+        StreamScope scope(s);
+        const bool dev = on_device != 0;
+        DevBuf<float> bin, bout;
+        const float* i = dev_in(in, n, dev, bin, s);
+        float* o = out;
+        if (!dev) {
+            bout.alloc(n);
+            o = bout.p;
+        }
+        launch_to_file_order(i, nx, ny, nz, nt, o, s);
+        if (!dev && n) SF_HIP(hipMemcpyAsync(out, o, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        SF_HIP(hipStreamSynchronize(s));
+    });
+}
+
 int shorefit_microstructure(const float* coefficients, const uint8_t* mask, int64_t nx, int64_t ny, int64_t nz,
                             uint32_t radial_order, double zeta, const float* directions, int32_t units,
                             float outlier_factor, float* rtop, float* rtap, float* rtpp, float* msd, float* qiv,

@@ -343,4 +343,11 @@ void launch_micro_thresholds(const float* raw, int64_t V, float factor, MicroThr
 void launch_micro_scatter(const float* raw, const int64_t* vox, int64_t V, const MicroThresholds* d_th,
                           float* const* maps, hipStream_t s);
 
+// NIfTI storage order <-> voxel-major (volume.hip). raw holds the file's
+// elements (NIfTI datatype code, native byte order).
+void launch_to_voxel_major(const void* raw, int datatype, int64_t nx, int64_t ny, int64_t nz, int64_t nt, float slope,
+                           float inter, float* out, hipStream_t s);
+void launch_to_file_order(const float* in, int64_t nx, int64_t ny, int64_t nz, int64_t nt, float* out,
+                          hipStream_t s);
+
 }  // namespace shorefit

@@ -116,10 +116,10 @@ def _sidecar(out: L.ShorefitOutput, spec: L.ShorefitSpec, regularization, alpha_
             "l2-anchored": {"kind": "path-l2-anchored", "path": {"n": path_n_alphas, "eps": eps}, "slack": slack},
         }[alpha_mode]
         if alpha_mode == "fixed":
-            chosen = {"kind": "global", "alpha": alpha}
+            chosen = {"scope": "global", "alpha": alpha}
         else:
             med, p10, p90 = alpha_distribution
-            chosen = {"kind": "per-voxel", "median": med, "p10": p10, "p90": p90}
+            chosen = {"scope": "per-voxel", "median": med, "p10": p10, "p90": p90}
         solver = {"kind": "fista", "alpha_strategy": strat, "chosen_alpha": chosen, "non_negative": non_negative,
                   "max_iter": max_iter, "tol": tol}
     else:
@@ -135,8 +135,12 @@ def _sidecar(out: L.ShorefitOutput, spec: L.ShorefitSpec, regularization, alpha_
         "n_coefficients": n_coeffs_,
         "bvec_frame": bvec_frame,
         "dwi_shells": [float(out.dwi_shells[i]) for i in range(min(out.n_shells, 32))],
-        "provenance": None,
     }
+
+
+def _is_device_tensor(x) -> bool:
+    """True for a torch tensor on a GPU (the fit then runs on it in place)."""
+    return hasattr(x, "data_ptr") and bool(getattr(x, "is_cuda", False))
 
 
 def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_threshold=50.0,
@@ -168,40 +172,68 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
     spec = _spec(radial_order, zeta, regularization, alpha_mode, alpha, alpha_ratio, path_n_alphas, path_eps,
                  slack, max_iter, tol, non_negative, lambda_n, lambda_l, diagnostics, allow_single_shell,
                  precision, refresh_every)
-    vol = np.asarray(data)
-    if vol.ndim != 4:
-        raise ValueError(f"data must be 4-D (X, Y, Z, N), got shape {vol.shape}")
-    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    on_dev = _is_device_tensor(data)
+    if on_dev:
+        import torch
+        vol = data
+        if vol.dim() != 4:
+            raise ValueError(f"data must be 4-D (X, Y, Z, N), got shape {tuple(vol.shape)}")
+        vol = vol.to(torch.float32).contiguous()
+        dev = vol.device
+    else:
+        vol = np.asarray(data)
+        if vol.ndim != 4:
+            raise ValueError(f"data must be 4-D (X, Y, Z, N), got shape {vol.shape}")
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
     g, keep = L.gradients(bvals, bvecs, big_delta, small_delta, b0_threshold)
     if vol.shape[3] != g.n_grads:
         raise ValueError(f"data has {vol.shape[3]} volumes but the gradient table has {g.n_grads}")
-    m = np.asarray(mask)
-    if m.shape != vol.shape[:3]:
-        raise ValueError(f"mask shape {m.shape} does not match data spatial shape {vol.shape[:3]}")
-    m = np.ascontiguousarray(m.astype(np.uint8))
+    if on_dev:
+        m = mask if _is_device_tensor(mask) else torch.from_numpy(np.asarray(mask))
+        if tuple(m.shape) != tuple(vol.shape[:3]):
+            raise ValueError(f"mask shape {tuple(m.shape)} does not match data spatial shape {tuple(vol.shape[:3])}")
+        m = m.to(device=dev, dtype=torch.uint8).contiguous()
+    else:
+        m = np.asarray(mask)
+        if m.shape != vol.shape[:3]:
+            raise ValueError(f"mask shape {m.shape} does not match data spatial shape {vol.shape[:3]}")
+        m = np.ascontiguousarray(m.astype(np.uint8))
     nx, ny, nz, _ = vol.shape
     K = n_coeffs(radial_order) if radial_order % 2 == 0 and radial_order > 0 else 0
     shp = (nx, ny, nz)
-    coefs = np.zeros(shp + (K,), np.float32)
+
+    def zeros(shape, dt):
+        if on_dev:
+            tdt = {np.float32: torch.float32, np.uint32: torch.int32, np.uint8: torch.uint8}[dt]
+            return torch.zeros(shape, dtype=tdt, device=dev)
+        return np.zeros(shape, dt)
+
+    def addr(a):
+        return a.data_ptr() if on_dev else a.ctypes.data
+
+    coefs = zeros(shp + (K,), np.float32)
     diag = bool(diagnostics)
     l1 = regularization == "l1"
-    r2 = np.zeros(shp, np.float32) if diag else None
-    res = np.zeros(shp, np.float32) if diag else None
-    its = np.zeros(shp, np.uint32) if diag else None
-    kind = np.zeros(shp, np.uint8) if diag else None
-    amap = np.zeros(shp, np.float32) if (diag and l1) else None
-    bic = np.zeros(shp, np.float32) if (diag and l1) else None
+    r2 = zeros(shp, np.float32) if diag else None
+    res = zeros(shp, np.float32) if diag else None
+    its = zeros(shp, np.uint32) if diag else None
+    kind = zeros(shp, np.uint8) if diag else None
+    amap = zeros(shp, np.float32) if (diag and l1) else None
+    bic = zeros(shp, np.float32) if (diag and l1) else None
     # the Tikhonov reference RSS map of the L2-anchored strategy (src/fit.rs:384-386)
-    rss_l2 = np.zeros(shp, np.float32) if (diag and l1 and alpha_mode == "l2-anchored") else None
+    rss_l2 = zeros(shp, np.float32) if (diag and l1 and alpha_mode == "l2-anchored") else None
     out = L.ShorefitOutput()
-    out.coefficients = L.ptr(coefs).value
+    out.coefficients = addr(coefs)
     for name, arr in (("r2", r2), ("residual_l2", res), ("iterations", its), ("regularization_kind", kind),
                       ("alpha", amap), ("bic", bic), ("rss_l2", rss_l2)):
-        setattr(out, name, None if arr is None else arr.ctypes.data)
-    v = L.ShorefitVolume(vol.ctypes.data, m.ctypes.data, nx, ny, nz, 0, 0)
+        setattr(out, name, None if arr is None else addr(arr))
+    v = L.ShorefitVolume(addr(vol), addr(m), nx, ny, nz, 1 if on_dev else 0, 0)
+    stream = torch.cuda.current_stream(dev).cuda_stream if on_dev else None
     if graddev is None:
-        L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), None))
+        L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), stream))
     else:
+        if on_dev:
+            raise ValueError("graddev fits take host arrays")
         gd, _keep_gd = _graddev_struct(graddev, (nx, ny, nz), g.n_grads, bvecs_vox, frame_rotation,
                                        graddev_variant)
         L.check(L.lib().shorefit_fit_graddev(C.byref(g), C.byref(v), C.byref(gd), C.byref(spec), C.byref(out),
@@ -217,7 +249,8 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
     }
     if diag:
         # src/fit.rs:810-822 — rmse = residual_l2 / sqrt(n_grads)
-        result["rmse"] = (res / np.float32(np.sqrt(np.float32(g.n_grads)))).astype(np.float32)
+        denom = np.float32(np.sqrt(np.float32(g.n_grads)))
+        result["rmse"] = res / float(denom) if on_dev else (res / denom).astype(np.float32)
         result["r2"] = r2
         result["residual_l2"] = res
         result["iterations"] = its

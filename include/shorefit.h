@@ -291,6 +291,23 @@ int shorefit_microstructure(const float* coefficients, const uint8_t* mask, int6
                             float outlier_factor, float* rtop, float* rtap, float* rtpp, float* msd, float* qiv,
                             float* ng, int32_t on_device, void* stream);
 
+/* NIfTI storage order -> voxel-major f32. raw holds the elements of a 4D
+ * series as the file stores them (volume after volume, x fastest; native byte
+ * order; NIfTI-1 datatype code 2, 4, 8, 16, 64, 256, 512 or 768); out
+ * receives (nx, ny, nz, nt) float32 in C order, each voxel's samples
+ * contiguous. With scl_slope != 0 every value is f32(raw) * scl_slope +
+ * scl_inter in f32. raw and out are both host or both device pointers
+ * (on_device). Replaces cs_dmri::io::dwi::read_dwi_volume / write_volume_block
+ * (src/io/dwi.rs:219-313). */
+int shorefit_volume_to_voxel_major(const void* raw, int32_t datatype, int64_t nx, int64_t ny, int64_t nz, int64_t nt,
+                                   float scl_slope, float scl_inter, float* out, int32_t on_device, void* stream);
+
+/* Voxel-major (nx, ny, nz, nt) float32 -> NIfTI storage order (volume after
+ * volume, x fastest), float32. Replaces the layout step of the coefficient
+ * NIfTI writer, CoefficientsFile::write (src/io/coeffs.rs:112-160). */
+int shorefit_volume_to_file_order(const float* in, int64_t nx, int64_t ny, int64_t nz, int64_t nt, float* out,
+                                  int32_t on_device, void* stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* shorefit_last_error(void);
 

@@ -148,10 +148,10 @@ def test_volume_layout_and_diagnostics(orc, mode):
         assert out["alpha_distribution"] is not None
         np.testing.assert_allclose(out["alpha_distribution"], ref["alpha_distribution"], rtol=1e-6)
         assert out["alpha"][1, 2, 0] == 0 and out["iterations"][1, 2, 0] == 0
-        assert out["sidecar"]["solver"]["chosen_alpha"]["kind"] == "per-voxel"
+        assert out["sidecar"]["solver"]["chosen_alpha"]["scope"] == "per-voxel"
     else:
         assert out["alpha_distribution"] is None
-        assert out["sidecar"]["solver"]["chosen_alpha"] == {"kind": "global", "alpha": 1e-3}
+        assert out["sidecar"]["solver"]["chosen_alpha"] == {"scope": "global", "alpha": 1e-3}
 
 
 def test_tikhonov_volume(orc):

@@ -1,0 +1,136 @@
+"""GPU parity of the on-disk data path.
+
+* Layout conversion on the device (``shorefit_volume_to_voxel_major`` /
+  ``shorefit_volume_to_file_order``): bit-identical to numpy's transpose and
+  type conversion for every supported NIfTI type, with and without
+  scl_slope / scl_inter (f32(raw) * slope + inter in f32), on odd shapes and
+  on a partial final tile, from host and from device memory.
+* The reference's read_dwi_volume test (src/io/dwi.rs:466-485) restated: a
+  float64 series whose values encode their (x, y, z, t) reads back at the
+  same indices, from .nii and .nii.gz, 37 volumes.
+* ``fit_files``: files written for a small volume, loaded and fitted, give
+  exactly the coefficients and maps of ``shore_fit`` on the same arrays; the
+  coefficient file reads back bit for bit with the sidecar the fit returned;
+  the device path (device="cuda:0") gives the same coefficients.
+"""
+
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import cs_dmri_amd as sf  # noqa: E402
+from cs_dmri_amd import io as sio  # noqa: E402
+from cs_dmri_amd import nifti as nf  # noqa: E402
+from cs_dmri_amd import workloads as W  # noqa: E402
+
+TYPES = [(2, np.uint8), (256, np.int8), (4, np.int16), (512, np.uint16), (8, np.int32), (768, np.uint32),
+         (16, np.float32), (64, np.float64)]
+
+
+def _expected(raw_file_order, slope, inter):
+    v = raw_file_order.astype(np.float32)
+    if slope != 0.0:
+        v = v * np.float32(slope) + np.float32(inter)
+    return np.ascontiguousarray(v.transpose(3, 2, 1, 0))
+
+
+@pytest.mark.parametrize("code,dt", TYPES)
+@pytest.mark.parametrize("scl", [(0.0, 0.0), (0.37, -12.5)])
+def test_to_voxel_major_types_and_scaling(tmp_path, code, dt, scl):
+    rng = np.random.default_rng(code)
+    shape = (67, 5, 3, 70)  # x and t each span one full and one partial 64-wide tile
+    info = np.iinfo(dt) if np.issubdtype(dt, np.integer) else None
+    lo, hi = (max(info.min, -30000), min(info.max, 30000)) if info else (-1e4, 1e4)
+    raw = rng.uniform(lo, hi, size=shape[::-1]).astype(dt)
+    h = nf.new_header(shape, datatype=code, bitpix=8 * np.dtype(dt).itemsize)
+    h["scl_slope"], h["scl_inter"] = scl
+    p = tmp_path / "v.nii"
+    nf.save(p, h, raw)
+    img = nf.load(p)
+    want = _expected(raw, *img.scl)
+    got = sio.to_voxel_major(img)
+    assert got.dtype == np.float32 and got.shape == shape
+    np.testing.assert_array_equal(got, want)
+    got_dev = sio.to_voxel_major(img, device="cuda:0")
+    np.testing.assert_array_equal(got_dev.cpu().numpy(), want)
+
+
+def test_file_order_round_trip():
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(9, 70, 4, 131)).astype(np.float32)
+    f = sio.to_file_order(v)
+    np.testing.assert_array_equal(f, v.transpose(3, 2, 1, 0))
+    import torch
+    f2 = sio.to_file_order(torch.from_numpy(v).to("cuda:0"))
+    np.testing.assert_array_equal(f2, f)
+
+
+@pytest.mark.parametrize("name", ["coded.nii", "coded.nii.gz"])
+def test_read_dwi_volume_keeps_axes_in_standard_layout(tmp_path, name):
+    nx, ny, nz, nt = 5, 6, 7, 37
+    x, y, z, t = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), np.arange(nt), indexing="ij")
+    code = (x + 10 * y + 100 * z + 1000 * t).astype(np.float64)
+    nf.save(tmp_path / name, nf.new_header((nx, ny, nz, nt), datatype=64, bitpix=64), code.transpose(3, 2, 1, 0))
+    data = sio.read_dwi_volume(tmp_path / name)
+    assert data.shape == (nx, ny, nz, nt) and data.flags["C_CONTIGUOUS"]
+    np.testing.assert_array_equal(data.astype(np.float64), code)
+
+
+def _write_inputs(tmp_path, data, g, mask, affine):
+    dwi = tmp_path / "dwi.nii.gz"
+    nf.save(dwi, nf.new_header(data.shape, affine, datatype=16, bitpix=32), data.transpose(3, 2, 1, 0))
+    (tmp_path / "dwi.bval").write_text(" ".join(f"{b:.17g}" for b in g.bvals) + "\n")
+    (tmp_path / "dwi.bvec").write_text("\n".join(" ".join(f"{v:.17g}" for v in g.bvecs[:, i]) for i in range(3)) + "\n")
+    m = tmp_path / "mask.nii.gz"
+    nf.save(m, nf.new_header(mask.shape, affine, datatype=2, bitpix=8), mask.astype(np.uint8).transpose(2, 1, 0))
+    return dwi, tmp_path / "dwi.bval", tmp_path / "dwi.bvec", m
+
+
+def test_fit_files_matches_shore_fit(tmp_path):
+    wl = W.config1(shape=(6, 5, 4))
+    g = wl.gtab
+    data = np.ascontiguousarray(wl.data)
+    mask = np.ones(data.shape[:3], bool)
+    mask[0, 0, :2] = False
+    aff = np.array([[-1.7, 0, 0, 10.0], [0, 1.7, 0, -20.0], [0, 0, 2.0, 5.0], [0, 0, 0, 1]])
+    dwi, bval, bvec, m = _write_inputs(tmp_path, data, g, mask, aff)
+    kw = dict(radial_order=6, alpha_mode="fixed", alpha=1e-3, max_iter=200, diagnostics=True)
+    out = tmp_path / "out" / "coef.nii.gz"
+    side = sio.fit_files(dwi, bval, bvec, out, mask=m, bvec_frame="world-ras", big_delta=g.big_delta,
+                         small_delta=g.small_delta, **kw)
+    # the same fit on the arrays, with the bvecs rotated into world RAS
+    rot = sio.affine_rotation(aff)
+    ref = sf.shore_fit(data, g.bvals, sio.rotate_bvecs(g.bvecs, rot), mask, big_delta=g.big_delta,
+                       small_delta=g.small_delta, bvec_frame="world-ras", **kw)
+    coef, meta = sio.read_coefficients(out)
+    np.testing.assert_array_equal(coef, ref["coefficients"])
+    assert meta == json.loads(json.dumps(ref["sidecar"])) and meta == side
+    assert meta["bvec_frame"] == "world-ras" and meta["n_coefficients"] == 72
+    assert nf.load(out).affine().tolist() == nf.load(dwi).affine().tolist()
+    for key in ("r2", "rmse", "alpha", "bic"):
+        mp = nf.load(sio.sibling_path(out, f"_{key}.nii.gz"))
+        np.testing.assert_array_equal(np.asarray(mp.data).transpose(2, 1, 0), ref[key])
+    sp = np.asarray(nf.load(sio.sibling_path(out, "_sparsity.nii.gz")).data).transpose(2, 1, 0)
+    np.testing.assert_array_equal(sp, sio.sparsity_map(ref["coefficients"], mask))
+    with pytest.raises(FileExistsError):
+        sio.fit_files(dwi, bval, bvec, out, mask=m, big_delta=g.big_delta, small_delta=g.small_delta, **kw)
+    # the whole path on the device
+    out2 = tmp_path / "coef_dev.nii"
+    sio.fit_files(dwi, bval, bvec, out2, mask=m, bvec_frame="world-ras", big_delta=g.big_delta,
+                  small_delta=g.small_delta, device="cuda:0", **kw)
+    np.testing.assert_array_equal(sio.read_coefficients(out2)[0], ref["coefficients"])
+
+
+def test_load_dwi_default_mask_and_image_axis(tmp_path):
+    wl = W.config1(shape=(4, 4, 2))
+    g = wl.gtab
+    data = np.ascontiguousarray(wl.data)
+    data[0, 0, 0] = 0.0  # a background voxel
+    dwi, bval, bvec, _ = _write_inputs(tmp_path, data, g, np.ones(data.shape[:3], bool), np.eye(4))
+    d = sio.load_dwi(dwi, bval, bvec, None, bvec_frame="image-axis")
+    np.testing.assert_array_equal(d.data, data)
+    np.testing.assert_array_equal(d.bvecs, g.bvecs)
+    assert not d.mask[0, 0, 0] and d.mask.sum() == data.shape[0] * data.shape[1] * data.shape[2] - 1
