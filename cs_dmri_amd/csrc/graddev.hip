@@ -27,6 +27,8 @@
 //
 // FP64 is used where the shared-design path uses it: design, Gram matrix,
 // right-hand side, Lipschitz estimate, residual. The iteration itself is FP32.
+#include <type_traits>
+
 #include "common.h"
 #include "special.h"
 
@@ -162,78 +164,140 @@ __device__ inline RowGeom gd_row(const GdTable& t, const double (&T)[9], int i, 
     return g;
 }
 
-// Radial factors of one row for one even l: pairs (n, l), n = l..R, from the
-// Laguerre recurrence in n - l (the recurrence of laguerre_d), times
-// kappa(n, l) e^(-u/2) u^(l/2) (src/basis/shore.rs:56-70). Pair (n, l) is
-// column P_before(n) + l / 2 of the factor row, P_before(n) = sum over n' < n
-// of (n'/2 + 1).
-template <int R>
-__device__ inline void radial_task(const GdBasis& B, int l, double u, double e, double* fac) {
-    const double a = l + 0.5;
-    double ep = e;
-    for (int j = 0; j < l / 2; ++j) ep *= u;
-    double lp = 0.0, lc = 1.0;
-    int p = l + (l / 2) * (l / 2 - 1) + l / 2;
-    for (int n = l; n <= R; ++n) {
-        const int k = n - l;
-        if (k == 1) {
-            lp = 1.0;
-            lc = 1.0 + a - u;
-        } else if (k >= 2) {
-            const double nw = ((2.0 * (k - 1) + 1.0 + a - u) * lc - ((k - 1) + a) * lp) / double(k);
-            lp = lc;
-            lc = nw;
-        }
-        fac[p] = B.kappa[p] * lc * ep;
-        p += n / 2 + 1;
+// Compile-time loop: f(std::integral_constant<int, i>) for i = B .. E-1.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
     }
 }
 
-// Angular factors of one row for one order k = |m|: P_l^k(cos phi) for
-// l = k..R by the upward recurrence of legendre_d, times the BrainSuite
-// normalisation and cos / sin of k theta (src/basis/shore.rs:76-115), into the
-// slots l - k and l + k of every even l >= max(k, 2) (and l = 0 for k = 0).
+// Basis index set (src/basis/shore.rs:29-44): n = 0..R, even l <= n, 2l + 1
+// slots (m = -l..l) per (n, l). Pair (n, l) is entry pair_c(n, l) of the
+// radial factors, its first column col0_c(n, l); slot s of even l is entry
+// ang_c(l, s) of the angular factors.
+constexpr int pair_c(int n, int l) {
+    int p = 0;
+    for (int j = 0; j < n; ++j) p += j / 2 + 1;
+    return p + l / 2;
+}
+constexpr int col0_c(int n, int l) {
+    int c = 0;
+    for (int j = 0; j < n; ++j)
+        for (int q = 0; q <= j; q += 2) c += 2 * q + 1;
+    for (int q = 0; q < l; q += 2) c += 2 * q + 1;
+    return c;
+}
+constexpr int ang_c(int l, int slot) { return l * (l - 1) / 2 + slot; }
+
+// Tile t of the upper triangle (I <= J) of a KT x KT tile grid in row-major
+// order: its block row and column; (0, 0) for t past the last tile.
+constexpr int tile_row(int KT, int t) {
+    int I = 0;
+    while (I < KT && t >= KT - I) t -= KT - I++;
+    return I < KT ? I : 0;
+}
+constexpr int tile_col(int KT, int t) {
+    int I = 0;
+    while (I < KT && t >= KT - I) t -= KT - I++;
+    return I < KT ? I + t : 0;
+}
+
+// One design row of A_v from its geometry, written to arow[0 .. K-1]
+// (src/basis/shore.rs:122-178): column (n, l, slot) is
+//     kappa(n, l) L_{n-l}^{(l+1/2)}(u) u^(l/2) e^(-u/2) * c_l^slot P_l^k(cos phi) trig(k theta)
+// with k = |l - slot| and trig = cos for slot < l, sin for slot > l
+// (src/basis/shore.rs:56-115). The loops are unrolled at compile time, so the
+// factors live in registers and every column index is a constant:
+//   radial  for each even l, the Laguerre recurrence in n - l (the recurrence
+//           of laguerre_d) over n = l..R;
+//   angular for k = 0..R, cos / sin (k theta) and P_k^k by carrying the
+//           k-1 values one step, then the upward Legendre recurrence in l (the
+//           recurrence of legendre_d); each even l >= max(k, 2) (and l = 0 for
+//           k = 0) emits its columns for every n >= l.
+// Divisions by the recurrence index are multiplications by its reciprocal.
+// kap[p] = kappa of pair p and angc[a] = normalisation of angular entry a are
+// read from LDS (uniform addresses): held in scalar registers, the 70-128
+// constants would not fit.
 template <int R>
-__device__ inline void angular_task(const GdBasis& B, int k, double x, double ct, double st, double* fac) {
-    double ck = 1.0, sk = 0.0;
-    for (int j = 0; j < k; ++j) {
-        const double cn = ck * ct - sk * st;
-        sk = sk * ct + ck * st;
-        ck = cn;
-    }
-    auto emit = [&](int l, double pv) {
-        if (l & 1) return;
-        if (l == 0) {
-            fac[0] = B.ang_c[0];
-            return;
-        }
-        const int base = l * (l - 1) / 2;
-        if (k == 0) {
-            fac[base + l] = B.ang_c[base + l] * pv;
-        } else {
-            fac[base + l - k] = B.ang_c[base + l - k] * pv * ck;
-            fac[base + l + k] = B.ang_c[base + l + k] * pv * sk;
-        }
+__device__ __forceinline__ void build_row(const double* kap, const double* angc, const RowGeom& g,
+                                          double* __restrict__ arow) {
+    constexpr int P = n_pairs_c(R);
+    double rad[P];
+    static_for<0, R / 2 + 1>([&](auto lh) {
+        constexpr int l = 2 * decltype(lh)::value;
+        constexpr double a = l + 0.5;
+        double ep = g.e;
+#pragma unroll
+        for (int j = 0; j < l / 2; ++j) ep *= g.u;
+        double lp = 0.0, lc = 1.0;
+        static_for<l, R + 1>([&](auto nn) {
+            constexpr int n = decltype(nn)::value, k = n - l;
+            if constexpr (k == 1) {
+                lp = 1.0;
+                lc = 1.0 + a - g.u;
+            } else if constexpr (k >= 2) {
+                constexpr double c1 = 2.0 * (k - 1) + 1.0 + a, c2 = (k - 1) + a, rk = 1.0 / k;
+                const double nw = ((c1 - g.u) * lc - c2 * lp) * rk;
+                lp = lc;
+                lc = nw;
+            }
+            rad[pair_c(n, l)] = kap[pair_c(n, l)] * lc * ep;
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+
+    // the columns of angular slot `slot` of even l, value av
+    auto emit = [&](auto lc_, auto slot_, double av) {
+        constexpr int l = decltype(lc_)::value, slot = decltype(slot_)::value;
+        static_for<l, R + 1>([&](auto nn) {
+            constexpr int n = decltype(nn)::value;
+            arow[col0_c(n, l) + slot] = rad[pair_c(n, l)] * av;
+        });
     };
-    double pmm = 1.0;
-    if (k > 0) {
-        const double sq = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
-        double f = 1.0;
-        for (int i = 0; i < k; ++i) {
-            pmm *= -f * sq;
-            f += 2.0;
+    const double x = g.cp;
+    const double sq = sqrt(fmax((1.0 - x) * (1.0 + x), 0.0));
+    double ck = 1.0, sk = 0.0, pmm = 1.0;
+    static_for<0, R + 1>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        if constexpr (k > 0) {
+            const double cn = ck * g.ct - sk * g.st;
+            sk = sk * g.ct + ck * g.st;
+            ck = cn;
+            pmm *= -(2.0 * k - 1.0) * sq;
         }
-    }
-    emit(k, pmm);
-    if (k + 1 > R) return;
-    double p1 = x * (2.0 * k + 1.0) * pmm;
-    emit(k + 1, p1);
-    for (int ll = k + 2; ll <= R; ++ll) {
-        const double pl = ((2.0 * ll - 1.0) * x * p1 - (ll + k - 1.0) * pmm) / double(ll - k);
-        pmm = p1;
-        p1 = pl;
-        emit(ll, pl);
-    }
+        // P_l^k for l = k..R; emits even l
+        auto at = [&](auto ll, double pv) {
+            constexpr int l = decltype(ll)::value;
+            if constexpr (l == 0) {
+                emit(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{}, angc[0]);
+            } else if constexpr (l % 2 == 0) {
+                if constexpr (k == 0) {
+                    emit(ll, std::integral_constant<int, l>{}, angc[ang_c(l, l)] * pv);
+                } else {
+                    emit(ll, std::integral_constant<int, l - k>{}, angc[ang_c(l, l - k)] * pv * ck);
+                    emit(ll, std::integral_constant<int, l + k>{}, angc[ang_c(l, l + k)] * pv * sk);
+                }
+            }
+        };
+        at(std::integral_constant<int, k>{}, pmm);
+        if constexpr (k + 1 <= R) {
+            double p0 = pmm, p1 = x * (2.0 * k + 1.0) * pmm;
+            at(std::integral_constant<int, k + 1>{}, p1);
+            static_for<k + 2, R + 1>([&](auto ll) {
+                constexpr int l = decltype(ll)::value;
+                constexpr double c1 = 2.0 * l - 1.0, c2 = l + k - 1.0, rl = 1.0 / (l - k);
+                const double pl = (c1 * x * p1 - c2 * p0) * rl;
+                p0 = p1;
+                p1 = pl;
+                at(ll, pl);
+            });
+        }
+        // keeps the scheduler from hoisting every constant of the row at
+        // once (which would not fit in registers)
+        __builtin_amdgcn_sched_barrier(0);
+    });
 }
 
 __device__ inline void load_T(const float* field, int64_t v, double (&T)[9]) {
@@ -276,12 +340,11 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
     gd_gram_kernel(const GdBasis B, const GdTable tab, const float* __restrict__ data, int64_t sig_stride,
                    const float* __restrict__ field, const int64_t* __restrict__ vox, int64_t n,
                    float* __restrict__ Gws, double* __restrict__ bws, double* __restrict__ sstat) {
-    constexpr int K = n_coeffs_c(R), P = n_pairs_c(R), Q = n_ang_c(R), PQ = P + Q;
-    constexpr int TR = R / 2 + 1 + R + 1;  // factor tasks per row: radial (one per even l), angular (one per |m|)
+    constexpr int K = n_coeffs_c(R), P = n_pairs_c(R), Q = n_ang_c(R);
     constexpr int KT = KP / 16, NT = KT * (KT + 1) / 2, TPW = (NT + WAVES - 1) / WAVES;
     constexpr int LDA = KP + 2;
     constexpr int NTH = 64 * WAVES;
-    static_assert(KP % 16 == 0 && KP >= K && RC % 4 == 0 && KP <= NTH, "tile shape");
+    static_assert(KP % 16 == 0 && KP >= K && RC % 4 == 0 && RC <= 64 && KP <= NTH && Q <= NTH, "tile shape");
     const int64_t slot = blockIdx.x;
     if (slot >= n) return;
     const int64_t v = vox[slot];
@@ -290,25 +353,30 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     extern __shared__ double smem[];
-    double* s_u = smem;
-    double* s_e = s_u + N;
-    double* s_cp = s_e + N;
-    double* s_ct = s_cp + N;
-    double* s_st = s_ct + N;
-    double* s_sig = s_st + N;
-    double* s_fac = s_sig + N;     // RC x PQ
-    double* s_A = s_fac + RC * PQ;  // RC x LDA
+    double* s_kap = smem;        // P
+    double* s_angc = s_kap + P;  // Q
+    double* s_sig = s_angc + Q;  // N
+    double* s_geo = s_sig + N;   // 5 x N: u, e, cos phi, cos theta, sin theta
+    double* s_A = s_geo + 5 * N; // RC x LDA
 
     double T[9];
     load_T(field, v, T);
+    // one thread per row: the effective table row and its geometry
     for (int i = tid; i < N; i += NTH) {
         const RowGeom g = gd_row(tab, T, i, B.zeta);
-        s_u[i] = g.u;
-        s_e[i] = g.e;
-        s_cp[i] = g.cp;
-        s_ct[i] = g.ct;
-        s_st[i] = g.st;
+        s_geo[i] = g.u;
+        s_geo[N + i] = g.e;
+        s_geo[2 * N + i] = g.cp;
+        s_geo[3 * N + i] = g.ct;
+        s_geo[4 * N + i] = g.st;
         s_sig[i] = double(data[v * sig_stride + i]);
+    }
+    if (tid < P) s_kap[tid] = B.kappa[tid];
+    if (tid < Q) s_angc[tid] = B.ang_c[tid];
+    // the padding columns K..KP-1 stay zero
+    for (int it = tid; it < RC * (KP - K); it += NTH) {
+        const int r = it / (KP - K);
+        s_A[r * LDA + K + (it - r * (KP - K))] = 0.0;
     }
     __syncthreads();
     if (w == 0) {
@@ -332,17 +400,14 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
     }
 
     // tiles of this wave: t = w + WAVES j in row-major order of the upper
-    // triangle (I <= J)
+    // triangle (I <= J); the slots past the last tile (at most one per wave
+    // for the shapes used) repeat tile (0, 0) and are not written, so the
+    // MFMA loop has no branches
     int tI[TPW], tJ[TPW];
 #pragma unroll
     for (int j = 0; j < TPW; ++j) {
-        int t = w + WAVES * j, I = 0;
-        while (I < KT && t >= KT - I) {
-            t -= KT - I;
-            ++I;
-        }
-        tI[j] = I;
-        tJ[j] = I + t;
+        tI[j] = tile_row(KT, w + WAVES * j);
+        tJ[j] = tile_col(KT, w + WAVES * j);
     }
     doublex4 acc[TPW];
 #pragma unroll
@@ -352,37 +417,46 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
     for (int i0 = 0; i0 < N; i0 += RC) {
         __syncthreads();  // previous chunk's A reads done
         const int rmax = min(RC, N - i0);
-        for (int t = tid; t < rmax * TR; t += NTH) {
-            const int r = t / TR, q = t - r * TR;
-            const int i = i0 + r;
-            if (q <= R / 2)
-                radial_task<R>(B, 2 * q, s_u[i], s_e[i], s_fac + r * PQ);
-            else
-                angular_task<R>(B, q - (R / 2 + 1), s_cp[i], s_ct[i], s_st[i], s_fac + r * PQ + P);
-        }
-        __syncthreads();
-        for (int it = tid; it < RC * KP; it += NTH) {
-            const int r = it / KP, c = it - r * KP;
-            double val = 0.0;
-            if (c < K && r < rmax) val = s_fac[r * PQ + B.col_pair[c]] * s_fac[r * PQ + P + B.col_ang[c]];
-            s_A[r * LDA + c] = val;
-        }
-        __syncthreads();
-        if (tid < KP)
-            for (int r = 0; r < rmax; ++r) bacc += s_A[r * LDA + tid] * s_sig[i0 + r];
-        const int ksteps = (rmax + 3) / 4;
-#pragma unroll
-        for (int kk = 0; kk < RC / 4; ++kk) {
-            if (kk >= ksteps) break;
-            const double* rowp = s_A + (4 * kk + (lane >> 4)) * LDA + (lane & 15);
-#pragma unroll
-            for (int j = 0; j < TPW; ++j) {
-                if (w + WAVES * j < NT) {
-                    const double a = rowp[16 * tI[j]], b = rowp[16 * tJ[j]];
-                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
-                }
+        // one thread per row of the chunk, in one wave (rows past the end
+        // are zero); the wave that builds rotates from chunk to chunk
+        if (w == (i0 / RC) % WAVES && lane < RC) {
+            double* arow = s_A + lane * LDA;
+            if (lane < rmax) {
+                const int i = i0 + lane;
+                const RowGeom g{s_geo[i], s_geo[N + i], s_geo[2 * N + i], s_geo[3 * N + i], s_geo[4 * N + i]};
+                build_row<R>(s_kap, s_angc, g, arow);
+            } else {
+                for (int c = 0; c < K; ++c) arow[c] = 0.0;
             }
         }
+        __syncthreads();
+        if (tid < KP) {
+#pragma unroll 8
+            for (int r = 0; r < rmax; ++r) bacc += s_A[r * LDA + tid] * s_sig[i0 + r];
+        }
+        // four rows per step (rows past rmax up to the next multiple of 4
+        // are zero)
+        const int ksteps = (rmax + 3) / 4;
+        // the tile list is a compile-time constant per wave, so every LDS
+        // read has an immediate offset and reads shared by two tiles of the
+        // wave are issued once
+        const double* rowp = s_A + (lane >> 4) * LDA + (lane & 15);
+        static_for<0, WAVES>([&](auto wc) {
+            constexpr int ww = decltype(wc)::value;
+            if (w != ww) return;
+            const double* rp = rowp;
+            for (int kk = 0; kk < ksteps; ++kk, rp += 4 * LDA) {
+                double a[TPW], b[TPW];
+                static_for<0, TPW>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    a[j] = rp[16 * tile_row(KT, ww + WAVES * j)];
+                    b[j] = rp[16 * tile_col(KT, ww + WAVES * j)];
+                });
+#pragma unroll
+                for (int j = 0; j < TPW; ++j)
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], acc[j], 0, 0, 0);
+            }
+        });
     }
 
     // D layout of the 16x16x4 f64 MFMA (measured on gfx950; it differs from
@@ -711,8 +785,7 @@ __global__ void gather_i64_kernel(const int64_t* __restrict__ src, const int64_t
 
 template <int R, int KP, int RC, int WAVES>
 size_t gram_lds(int N) {
-    constexpr int P = n_pairs_c(R), Q = n_ang_c(R);
-    return sizeof(double) * (size_t(6) * N + size_t(RC) * (P + Q) + size_t(RC) * (KP + 2));
+    return sizeof(double) * (size_t(n_pairs_c(R) + n_ang_c(R)) + size_t(6) * N + size_t(RC) * (KP + 2));
 }
 
 template <int R, int KP, int RC, int WAVES>
