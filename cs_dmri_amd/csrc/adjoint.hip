@@ -10,9 +10,10 @@
 //     w = G_SS^-1 g_S ,   dL/ds = A_S w ,   dL/dalpha = -N z^T w .
 //
 // One wave per signal: the active set by ballots, G_SS gathered from the FP64
-// Gram matrix into a packed lower triangle in LDS (sized by the batch's
-// largest active set), an in-place right-looking Cholesky (FP64), the two
-// triangular solves, then A_S w with A read from L2.
+// Gram matrix into a packed lower triangle in LDS (rows, sized by the batch's
+// largest active set), an in-place right-looking Cholesky (FP64) in which the
+// lanes own rows, the two triangular solves with the right-hand side in
+// registers, then A_S w with A read from L2.
 #include "common.h"
 
 namespace shorefit {
@@ -20,25 +21,24 @@ namespace {
 
 __device__ __forceinline__ int tri(int i) { return i * (i + 1) / 2; }
 
-// Row i and column j (j <= i) of entry e of a packed lower triangle.
-__device__ __forceinline__ void tri_decode(int e, int& i, int& j) {
-    int r = int((sqrtf(8.f * float(e) + 1.f) - 1.f) * 0.5f);
-    if (tri(r + 1) <= e) ++r;
-    if (tri(r) > e) --r;
-    i = r;
-    j = e - tri(r);
-}
-
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) x += __shfl_xor(x, m);
     return x;
 }
 
+// Value of lane `src` (wave-uniform) of a double.
+__device__ __forceinline__ double lane_value(double x, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+
 // One wave per signal: the workgroup barriers below synchronise a single
 // wave, so they cost next to nothing, and the LDS per signal is sized by the
 // batch's largest active set, so several signals share a CU.
 constexpr int kThreads = 64;
+constexpr int kRowsPerLane = 3;  // active sets up to 192 (LDS allows 190)
 
 __global__ void __launch_bounds__(kThreads)
     lasso_adjoint_kernel(const double* __restrict__ G, const double* __restrict__ A, int N, int K,
@@ -46,7 +46,6 @@ __global__ void __launch_bounds__(kThreads)
                          float* __restrict__ gsig, float* __restrict__ galpha, uint8_t* __restrict__ status) {
     extern __shared__ double sm[];
     __shared__ int s_idx[256];
-    __shared__ int s_fail;
     const int64_t v = blockIdx.x;
     if (v >= V) return;
     const int lane = threadIdx.x;
@@ -60,7 +59,6 @@ __global__ void __launch_bounds__(kThreads)
         if (act) s_idx[m + __popcll(bal & ((uint64_t(1) << lane) - 1))] = k0 + lane;
         m += __popcll(bal);
     }
-    if (lane == 0) s_fail = 0;
     __syncthreads();
 
     if (m == 0) {  // c = 0 is locally constant: zero gradients
@@ -72,40 +70,56 @@ __global__ void __launch_bounds__(kThreads)
         }
         return;
     }
-    double* L = sm;                // packed lower triangle of G_SS, m (m + 1) / 2
-    double* x = sm + tri(m);       // right-hand side, then the solution w
-    const int nt = tri(m);
-    for (int e = lane; e < nt; e += kThreads) {
-        int i, j;
-        tri_decode(e, i, j);
-        L[e] = G[size_t(s_idx[i]) * K + s_idx[j]];
+    double* L = sm;               // packed lower triangle of G_SS by rows, tri(m)
+    double* s_col = L + tri(m);   // the column being eliminated, then w
+    double* s_inv = s_col + m;    // 1 / L_jj
+    for (int i = 0; i < m; ++i) {
+        const double* grow = G + size_t(s_idx[i]) * K;
+        for (int k = lane; k <= i; k += kThreads) L[tri(i) + k] = grow[s_idx[k]];
     }
-    for (int i = lane; i < m; i += kThreads) x[i] = double(gcoef[v * K + s_idx[i]]);
-    __syncthreads();
 
-    // Cholesky G_SS = L L^T, column by column
+    // Cholesky G_SS = L L^T, column by column; every lane forms the pivot
+    // (a uniform value), lane t owns rows j + 1 + t + 64 q below it
+    bool fail = false;
     for (int j = 0; j < m; ++j) {
-        const int jj = tri(j) + j;
+        __syncthreads();  // the previous column's updates are visible
+        const double d = L[tri(j) + j];
+        if (!(d > 0.0)) {
+            fail = true;
+            break;
+        }
+        const double ljj = sqrt(d), inv = 1.0 / ljj;
+        for (int i = j + 1 + lane; i < m; i += kThreads) {
+            const double lij = L[tri(i) + j] * inv;
+            L[tri(i) + j] = lij;
+            s_col[i] = lij;
+        }
         if (lane == 0) {
-            const double d = L[jj];
-            if (!(d > 0.0)) s_fail = 1;
-            L[jj] = sqrt(d);
+            L[tri(j) + j] = ljj;
+            s_inv[j] = inv;
         }
         __syncthreads();
-        if (s_fail) break;
-        const double ljj = L[jj];
-        for (int i = j + 1 + lane; i < m; i += kThreads) L[tri(i) + j] /= ljj;
-        __syncthreads();
-        const int n = m - j - 1, cnt = tri(n);
-        for (int e = lane; e < cnt; e += kThreads) {
-            int ii, kk;
-            tri_decode(e, ii, kk);
-            const int i = j + 1 + ii, k = j + 1 + kk;
-            L[tri(i) + k] -= L[tri(i) + j] * L[tri(k) + j];
+        for (int i = j + 1 + lane; i < m; i += kThreads) {
+            const double lij = s_col[i];
+            double* row = L + tri(i);
+            int k = j + 1;
+            // eight at a time, every load issued before the stores (the
+            // column and the row share LDS, so loads after a store would
+            // wait for them)
+            for (; k + 7 <= i; k += 8) {
+                double cv[8], rv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    cv[u] = s_col[k + u];
+                    rv[u] = row[k + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) row[k + u] = fma(-lij, cv[u], rv[u]);
+            }
+            for (; k <= i; ++k) row[k] = fma(-lij, s_col[k], row[k]);
         }
-        __syncthreads();
     }
-    if (s_fail) {
+    if (fail) {
         const float nan = __int_as_float(0x7fc00000);
         if (gsig)
             for (int n = lane; n < N; n += kThreads) gsig[v * N + n] = nan;
@@ -115,32 +129,60 @@ __global__ void __launch_bounds__(kThreads)
         }
         return;
     }
-    // L y = g, then L^T w = y
+    __syncthreads();
+
+    // L y = g, then L^T w = y; lane t keeps entries t + 64 q in registers and
+    // each step broadcasts one solved entry
+    double xr[kRowsPerLane];
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const int i = lane + kThreads * q;
+        xr[q] = i < m ? double(gcoef[v * K + s_idx[i]]) : 0.0;
+    }
+    // entry j of the vector (read from each register, then chosen, so the
+    // registers are never indexed at run time)
+    auto entry = [&](int j) {
+        const int q = j / kThreads, src = j % kThreads;
+        const double e0 = lane_value(xr[0], src), e1 = lane_value(xr[1], src), e2 = lane_value(xr[2], src);
+        return q == 0 ? e0 : (q == 1 ? e1 : e2);
+    };
     for (int j = 0; j < m; ++j) {
-        if (lane == 0) x[j] /= L[tri(j) + j];
-        __syncthreads();
-        const double xj = x[j];
-        for (int i = j + 1 + lane; i < m; i += kThreads) x[i] -= L[tri(i) + j] * xj;
-        __syncthreads();
+        const double yj = entry(j) * s_inv[j];
+#pragma unroll
+        for (int q = 0; q < kRowsPerLane; ++q) {
+            const int i = lane + kThreads * q;
+            if (i > j && i < m) xr[q] = fma(-L[tri(i) + j], yj, xr[q]);
+            else if (i == j) xr[q] = yj;
+        }
     }
     for (int j = m - 1; j >= 0; --j) {
-        if (lane == 0) x[j] /= L[tri(j) + j];
-        __syncthreads();
-        const double xj = x[j];
-        for (int i = lane; i < j; i += kThreads) x[i] -= L[tri(j) + i] * xj;
-        __syncthreads();
+        const double wj = entry(j) * s_inv[j];
+#pragma unroll
+        for (int q = 0; q < kRowsPerLane; ++q) {
+            const int i = lane + kThreads * q;
+            if (i < j) xr[q] = fma(-L[tri(j) + i], wj, xr[q]);
+            else if (i == j) xr[q] = wj;
+        }
     }
 
     // dL/dalpha = -N z^T w
     double zw = 0.0;
-    for (int i = lane; i < m; i += kThreads) zw += (c[s_idx[i]] > 0.f ? x[i] : -x[i]);
+#pragma unroll
+    for (int q = 0; q < kRowsPerLane; ++q) {
+        const int i = lane + kThreads * q;
+        if (i < m) {
+            zw += c[s_idx[i]] > 0.f ? xr[q] : -xr[q];
+            s_col[i] = xr[q];
+        }
+    }
     zw = wave_sum(zw);
+    __syncthreads();
     // dL/ds = A_S w
     if (gsig)
         for (int n = lane; n < N; n += kThreads) {
             const double* arow = A + size_t(n) * K;
             double acc = 0.0;
-            for (int i = 0; i < m; ++i) acc = fma(arow[s_idx[i]], x[i], acc);
+            for (int i = 0; i < m; ++i) acc = fma(arow[s_idx[i]], s_col[i], acc);
             gsig[v * N + n] = float(acc);
         }
     if (lane == 0) {
@@ -166,7 +208,8 @@ __global__ void active_max_kernel(const float* __restrict__ coef, int K, int64_t
 
 }  // namespace
 
-size_t lasso_adjoint_lds(int m) { return sizeof(double) * (size_t(m) * (m + 1) / 2 + m); }
+// packed triangle, column buffer, 1 / L_jj
+size_t lasso_adjoint_lds(int m) { return sizeof(double) * (size_t(m) * (m + 1) / 2 + 2 * size_t(m)); }
 
 void launch_lasso_adjoint(const double* G, const double* A, int N, int K, const float* coef, const float* gcoef,
                           int64_t V, float* gsig, float* galpha, uint8_t* status, hipStream_t s) {
@@ -180,9 +223,9 @@ void launch_lasso_adjoint(const double* G, const double* A, int N, int K, const 
     SF_HIP(hipMemcpyAsync(&m_max, d_m.p, sizeof(int), hipMemcpyDeviceToHost, s));
     SF_HIP(hipStreamSynchronize(s));
     const size_t lds = lasso_adjoint_lds(std::max(m_max, 1));
-    if (lds > 150 * 1024)
+    if (m_max > kRowsPerLane * kThreads || lds > 150 * 1024)
         throw Error{SHOREFIT_E_UNSUPPORTED, "lasso adjoint: an active set of " + std::to_string(m_max) +
-                                                " coefficients exceeds LDS (at most 190)"};
+                                                " coefficients exceeds the kernel's limit (at most 192)"};
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(lasso_adjoint_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
