@@ -524,7 +524,7 @@ def adjoint_bench(args):
         "phase_ms": {"forward": round(fwd_ms, 3), "adjoint": round(adj_ms, 3)},
         "active_set": {"mean": float(nnz.mean()), "max": int(nnz.max())},
         "iterations": {"mean": float(its_np.mean()), "max": int(its_np.max())},
-        "adjoint_roofline": {"bound": "fp64 (LDS-resident Cholesky, barrier-bound)", "flops": adj_flops,
+        "adjoint_roofline": {"bound": "fp64 (Cholesky in registers for |S| <= 64, in LDS above; latency-bound)", "flops": adj_flops,
                              "tflops": round(adj_flops / (adj_ms * 1e-3) / 1e12, 3), "peak": 78.6},
         "status_not_pd": int((st != 0).sum()),
     }

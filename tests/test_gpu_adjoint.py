@@ -85,6 +85,30 @@ def test_adjoint_matches_fp64_oracle(orc, problem):
     np.testing.assert_allclose(ga.cpu().numpy(), ra, rtol=1e-6)
 
 
+def test_adjoint_register_and_lds_paths(orc, problem):
+    """Active sets of at most 64 coefficients go through the register kernel,
+    larger ones through the LDS kernel: both against the FP64 oracle on one
+    batch that holds both kinds and an empty active set."""
+    A_tr, _, S_tr, _ = problem
+    d = LassoDesign(A_tr)
+    S = torch.from_numpy(S_tr[:128]).to(DEV)
+    c_big, _ = d.forward(S, 1e-3, max_iter=500, tol=1e-6)
+    c_small, _ = d.forward(S, 2e-5, max_iter=500, tol=1e-6)
+    coef = torch.cat([c_big, c_small, torch.zeros_like(c_big[:1])])
+    nnz = (coef != 0).sum(dim=1).cpu().numpy()
+    assert nnz[-1] == 0 and (nnz[:-1] <= 64).any() and (nnz > 64).any(), (nnz.min(), nnz.max())
+    g = torch.randn(coef.shape, generator=torch.Generator().manual_seed(4)).to(DEV)
+    gs, ga, st = d.adjoint(coef, g)
+    assert (st.cpu().numpy() == 0).all()
+    assert not gs[-1].any() and float(ga[-1]) == 0.0
+    rs, ra = orc.lasso_adjoint(A_tr, coef[:-1].cpu().numpy().astype(np.float64),
+                               g[:-1].cpu().numpy().astype(np.float64))
+    err = _rel(gs[:-1].cpu().numpy().astype(np.float64), rs)
+    print(f"active sets {nnz[:-1].min()}-{nnz.max()}: grad_signal max rel {err.max():.2e}")
+    assert err.max() <= 1e-6
+    np.testing.assert_allclose(ga[:-1].cpu().numpy(), ra, rtol=1e-6)
+
+
 def test_adjoint_against_finite_differences(orc, problem):
     A_tr, _, S_tr, _ = problem
     rng = np.random.default_rng(5)
