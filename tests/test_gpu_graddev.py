@@ -132,7 +132,7 @@ def test_alpha_ratio_and_zero_voxel(orc):
     assert out["alpha_distribution"] is not None
 
 
-@pytest.mark.parametrize("order", [4, 6, 8])
+@pytest.mark.parametrize("order", [4, 6, 8, 10])
 def test_variants_and_chunks_agree(order):
     gt, data, field = _volume((4, 4, 2), seed=3)
     mask = np.ones(data.shape[:3], bool)
