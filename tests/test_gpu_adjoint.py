@@ -109,6 +109,28 @@ def test_adjoint_register_and_lds_paths(orc, problem):
     np.testing.assert_allclose(ga[:-1].cpu().numpy(), ra, rtol=1e-6)
 
 
+def test_adjoint_active_set_sizes(orc):
+    """Active sets of 1, 2, 63, 64 (the largest register-kernel case), 65 (the
+    smallest LDS-kernel case), 100 and all 145 coefficients in one batch, on a
+    well-conditioned Gaussian design; the adjoint depends only on the
+    support and signs of the coefficients, so they are drawn directly."""
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((232, 145))
+    d = LassoDesign(A)
+    sizes = np.repeat([1, 2, 63, 64, 65, 100, 145], 3)
+    coef = np.zeros((len(sizes), 145), np.float32)
+    for i, m in enumerate(sizes):
+        S = rng.choice(145, m, replace=False)
+        coef[i, S] = rng.choice([-1.0, 1.0], m) * rng.uniform(0.1, 1.0, m)
+    g = rng.standard_normal(coef.shape).astype(np.float32)
+    gs, ga, st = d.adjoint(torch.from_numpy(coef).to(DEV), torch.from_numpy(g).to(DEV))
+    assert (st.cpu().numpy() == 0).all()
+    rs, ra = orc.lasso_adjoint(A, coef.astype(np.float64), g.astype(np.float64))
+    err = _rel(gs.cpu().numpy().astype(np.float64), rs)
+    assert err.max() <= 1e-6, dict(zip(sizes.tolist(), err.tolist()))
+    np.testing.assert_allclose(ga.cpu().numpy(), ra, rtol=1e-6)
+
+
 def test_adjoint_against_finite_differences(orc, problem):
     A_tr, _, S_tr, _ = problem
     rng = np.random.default_rng(5)
