@@ -270,6 +270,21 @@ double* pinned_scalars() {
     return h.p;
 }
 
+// Momentum table (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112),
+// in FP64, rounded to FP32; max(max_iter, 1) entries, on the device.
+void upload_momentum(DevBuf<float>& mom, uint32_t max_iter, hipStream_t s) {
+    std::vector<float> host(std::max<uint32_t>(max_iter, 1));
+    double t = 1.0;
+    for (size_t k = 0; k < host.size(); ++k) {
+        const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
+        host[k] = float((t - 1.0) / tn);
+        t = tn;
+    }
+    mom.alloc(host.size());
+    SF_HIP(hipMemcpyAsync(mom.p, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice, s));
+    SF_HIP(hipStreamSynchronize(s));  // the host table goes out of scope
+}
+
 // FISTA set-up (src/solver/fista.rs:38-62) and right-hand sides. The caller
 // fills L's signal fields (data, vox, n_vox, N, sig_stride, K, Kp, Np),
 // precision, max_iter and path_n. In device order: G = A^T A, its largest
@@ -280,18 +295,8 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
     const int64_t V = L.n_vox;
     const bool table = L.precision != SHOREFIT_PREC_FP32 || L.path_n > 0;
     if (table) {
-        // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112), in FP64,
         // uploaded first, while the stream holds little work
-        std::vector<float> mom(std::max<uint32_t>(L.max_iter, 1));
-        double t = 1.0;
-        for (size_t k = 0; k < mom.size(); ++k) {
-            const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
-            mom[k] = float((t - 1.0) / tn);
-            t = tn;
-        }
-        w.mom.alloc(mom.size());
-        SF_HIP(hipMemcpyAsync(w.mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
-        SF_HIP(hipStreamSynchronize(s));  // the host table goes out of scope
+        upload_momentum(w.mom, L.max_iter, s);
         L.mom = w.mom.p;
     }
     d.G.alloc(size_t(d.K) * d.K);
@@ -565,16 +570,8 @@ void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, cons
             double* phase_ms = nullptr) {
     if (V <= 0) return;
     const int N = S.tab.N, K = S.B.K, KP = var.kp;
-    // momentum (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112)
-    std::vector<float> mom(std::max<uint32_t>(sp.max_iter, 1));
-    double t = 1.0;
-    for (size_t k = 0; k < mom.size(); ++k) {
-        const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
-        mom[k] = float((t - 1.0) / tn);
-        t = tn;
-    }
-    DevBuf<float> d_mom(mom.size());
-    SF_HIP(hipMemcpyAsync(d_mom.p, mom.data(), sizeof(float) * mom.size(), hipMemcpyHostToDevice, s));
+    DevBuf<float> d_mom;
+    upload_momentum(d_mom, sp.max_iter, s);
     const int64_t Vb = gd_chunk_voxels(var, V);
     DevBuf<float> G(size_t(Vb) * KP * KP);
     DevBuf<double> b(size_t(Vb) * KP), sstat(size_t(Vb) * 2);
@@ -623,7 +620,7 @@ void gd_run(const GdSetup& S, const GdVariant& var, const ShorefitSpec& sp, cons
     }
     if (phase_ms)
         for (auto& e : ev) (void)hipEventDestroy(e);
-    SF_HIP(hipStreamSynchronize(s));  // the host momentum table goes out of scope
+    SF_HIP(hipStreamSynchronize(s));  // the chunk buffers go out of scope
 }
 
 // The volume fit behind shorefit_fit and shorefit_fit_graddev: validation,
@@ -1181,6 +1178,13 @@ int shorefit_lasso_forward(const ShorefitDesign* h, const float* signals, int64_
         L.thr = float(h->ops.step * alpha_scalar);
         L.alpha_vox = alpha;
         L.warm = warm;
+        // persistent FP32 form: voxels of uneven convergence refill their slots
+        DevBuf<float> mom;
+        upload_momentum(mom, max_iter, s);
+        DevBuf<unsigned int> counter(1);
+        L.mom = mom.p;
+        L.counter = counter.p;
+        L.persist = true;
         launch_fista_rhs(L, s);
         launch_fista(L, s);
     });

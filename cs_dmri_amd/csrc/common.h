@@ -177,6 +177,9 @@ struct FistaLaunch {
     uint8_t* statusbuf;     // [n_vox]
     float* alphabuf;        // [n_vox]
     void* event_mid;        // optional hipEvent_t recorded right after the solve kernel
+    // FP32 only: persistent form (slots refilled from `counter`, momentum from
+    // `mom`) when alpha is fixed and no residual maps are asked for
+    bool persist;
     // alpha path strategies (path_n > 0): point i of voxel v is the FISTA
     // solve at alpha_i = alpha_max(v) path_f[i], warm-started from point i-1;
     // a selection pass then writes the chosen point's outputs.

@@ -489,6 +489,165 @@ void launch_rhs(const FistaLaunch& L, hipStream_t s) {
     SF_HIP(hipGetLastError());
 }
 
+// Persistent FP32 form (fixed alpha, scalar or per voxel; warm starts): the
+// iteration of fista_f32_kernel, but each of a wave's 16 voxel slots takes a
+// new voxel from a global counter as soon as its voxel stops, so a wave does
+// not idle until the slowest of 16 voxels has converged. The momentum of a
+// slot comes from the table of the FP64 recurrence (the values the plain
+// kernel computes), and every voxel's arithmetic is the plain kernel's, so
+// the coefficients and iteration counts are the same bit for bit.
+// Coefficients, iterations, convergence, kind and alpha are written as each
+// voxel stops (no residual maps: the caller uses the plain kernel for those).
+template <int MB, int WAVES, bool M_IN_LDS>
+__global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const FistaArgs a, unsigned int* counter) {
+    extern __shared__ floatx4 Ms[];
+    const int lane = threadIdx.x & 63;
+    const int h = lane >> 4, j = lane & 15;
+    if (M_IN_LDS) {
+        for (int i = threadIdx.x; i < MB * MB * 64; i += 64 * WAVES) Ms[i] = a.Mfrag[i];
+        __syncthreads();
+    }
+    auto load_m = [&](int idx) -> floatx4 {
+        if constexpr (M_IN_LDS) return Ms[idx];
+        else return a.Mfrag[idx];
+    };
+
+    floatx4 b[MB], x[MB], y[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) b[mb] = x[mb] = y[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    int64_t v = 0, out_idx = 0;
+    uint32_t its = 0;
+    float thr = a.thr;
+    double alpha_v = a.alpha;
+    bool active = false, zero_voxel = false;
+    bool need = true;  // this slot (all four lanes of column j agree) wants a voxel
+
+    // outputs of the slot's voxel (src/solver/fista.rs:113-143 diagnostics)
+    auto finish = [&](bool conv) {
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * mb + 4 * h + r;
+                if (k < a.K) a.coef[out_idx * a.K + k] = x[mb][r];
+            }
+        if (h == 0) {
+            if (a.iters) a.iters[out_idx] = its;
+            if (a.converged) a.converged[out_idx] = conv ? 1 : 0;
+            if (a.kind) a.kind[out_idx] = 1;
+            if (a.alpha_map) a.alpha_map[out_idx] = zero_voxel ? 0.f : float(alpha_v);
+        }
+    };
+
+    for (;;) {
+        if (__ballot(need) != 0ull) {
+            // slots without a voxel take the next one (lane j of the column
+            // draws it, the other three read it from lane j)
+            unsigned int nv = 0;
+            if (need && h == 0) nv = atomicAdd(counter, 1u);
+            nv = __shfl(nv, j);
+            bool load = false;
+            if (need) {
+                v = int64_t(nv);
+                if (v >= a.n_vox) {
+                    need = false;  // no work left for this slot
+                } else {
+                    load = true;
+                    out_idx = a.vox ? a.vox[v] : v;
+                }
+            }
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+                if (load) b[mb] = *reinterpret_cast<const floatx4*>(a.B + v * a.Kp + 16 * mb + 4 * h);
+            // alpha-ratio strategy: thr_v = ratio * max|b| (src/solver/alpha.rs:229-235)
+            float bmax = 0.f;
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) bmax = fmaxf(bmax, fabsf(b[mb][r]));
+            bmax = lane_max4(bmax);
+            if (load) {
+                thr = a.thr;
+                alpha_v = a.alpha;
+                zero_voxel = false;
+                if (a.ratio > 0.f) {
+                    zero_voxel = (bmax == 0.f);
+                    thr = a.ratio * bmax;
+                    alpha_v = double(bmax) / double(a.step) * double(a.ratio);
+                }
+                if (a.alpha_vox) {  // per-voxel alpha (fixed strategy): thr_v = alpha_v / L
+                    const float av = a.alpha_vox[v];
+                    thr = float(a.step64 * double(av));
+                    alpha_v = double(av);
+                }
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        // warm start x0 = y0 = warm, t = 1 (src/solver/fista.rs:76-80)
+                        const int k = 16 * mb + 4 * h + r;
+                        x[mb][r] = (a.warm && k < a.K) ? a.warm[out_idx * a.K + k] : 0.f;
+                        y[mb][r] = x[mb][r];
+                    }
+                its = 0;
+                if (zero_voxel || a.max_iter == 0) {
+                    finish(zero_voxel);  // nothing to iterate: take another voxel next round
+                } else {
+                    active = true;
+                    need = false;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (__ballot(need) == 0ull) break;
+            continue;
+        }
+        int moff = 0;
+        asm volatile("" : "+s"(moff));
+        floatx4 acc[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            floatx4 c = b[mb];
+#pragma unroll
+            for (int kb = 0; kb < MB; ++kb) {
+                const floatx4 mf = load_m(moff + (mb * MB + kb) * 64 + lane);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+            }
+            acc[mb] = c;
+        }
+        const float mom = a.mom[active ? its : 0];
+        float dd = 0.f, xx = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float val = y[mb][r] + acc[mb][r];
+                float xn = val - fminf(fmaxf(val, -thr), thr);  // strict soft threshold
+                if (a.nonneg) xn = fmaxf(xn, 0.f);
+                const float d = xn - x[mb][r];
+                dd = fmaf(d, d, dd);
+                xx = fmaf(xn, xn, xx);
+                if (active) {
+                    y[mb][r] = fmaf(mom, d, xn);
+                    x[mb][r] = xn;
+                }
+            }
+        dd = lane_sum4(dd);
+        xx = lane_sum4(xx);
+        if (active) {
+            ++its;
+            const float rel = sqrtf(dd) / fmaxf(sqrtf(xx), 1e-12f);
+            const bool conv = rel < a.tol;
+            if (conv || its >= a.max_iter) {
+                finish(conv);
+                active = false;
+                need = true;
+            }
+        }
+    }
+}
+
 template <int MB>
 void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
     constexpr int WAVES = (MB >= 16) ? 4 : 8;
@@ -504,6 +663,28 @@ void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
     SF_HIP(hipGetLastError());
 }
 
+
+template <int MB>
+void launch_f32_persist(const FistaArgs& args, unsigned int* counter, hipStream_t s) {
+    constexpr int WAVES = (MB >= 16) ? 4 : 8;
+    constexpr bool IN_LDS = (MB * MB * 64 * 16) <= 112 * 1024;
+    const size_t lds = IN_LDS ? size_t(MB) * MB * 64 * 16 : 0;
+    auto kern = fista_f32_persist_kernel<MB, WAVES, IN_LDS>;
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(lds));
+    int dev = 0, cus = 0, per_cu = 0;
+    SF_HIP(hipGetDevice(&dev));
+    SF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 64 * WAVES,
+                                                        lds));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t need = (args.n_vox + 16 * WAVES - 1) / (16 * WAVES);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(need, int64_t(cus) * per_cu));
+    SF_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(64 * WAVES), lds, s, args, counter);
+    SF_HIP(hipGetLastError());
+}
 
 // ---------------------------------------------------------------------------
 // Mixed-precision persistent form.
@@ -1248,9 +1429,17 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
         }
         return;
     }
+    // persistent FP32 form: fixed alpha, no path, no residual maps, a work
+    // counter and a momentum table supplied
+    const bool persist = L.persist && L.path_n == 0 && L.ratio == 0.f && L.counter && L.mom && !L.resid &&
+                         !L.r2 && !L.bic;
     switch (L.Kp / 16) {
 #define SF_F32(MB)                                                  \
     case MB:                                                        \
+        if (persist) {                                              \
+            launch_f32_persist<MB>(a, L.counter, s);                \
+            break;                                                  \
+        }                                                           \
         launch_f32<MB>(a, L.n_vox, s);                              \
         if (L.event_mid) SF_HIP(hipEventRecord(static_cast<hipEvent_t>(L.event_mid), s)); \
         if (L.path_n > 0) launch_path_select<MB>(a, s);             \

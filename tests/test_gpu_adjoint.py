@@ -4,6 +4,7 @@ Function, and the held-out alpha search).
 
 Tolerances:
 * forward with a per-voxel alpha equal to a scalar alpha: bit-identical;
+  the persistent forward kernel against the plain FP32 kernel: bit-identical;
   against the FP64 oracle at equal iteration counts: 1e-5 per voxel;
 * adjoint against the FP64 oracle adjoint on the same coefficients: 1e-6
   relative per voxel (both FP64 Cholesky on the same active set);
@@ -67,6 +68,22 @@ def test_forward_per_voxel_alpha_and_warm_start(orc, problem):
     # restarting from a converged fit needs far fewer iterations
     _, i5 = d.forward(S, a, warm=c1, max_iter=300, tol=1e-6)
     assert float(i5.float().median()) < 0.5 * float(i1.float().median())
+
+
+def test_forward_persistent_matches_plain_kernel(problem):
+    """The lasso forward runs the persistent FP32 kernel (slots refilled as
+    voxels stop); fista_batch on the same design runs the plain tile kernel.
+    Same arithmetic per voxel: identical coefficients and iteration counts."""
+    import cs_dmri_amd as sf
+    A_tr, _, S_tr, _ = problem
+    d = LassoDesign(A_tr)
+    S = np.ascontiguousarray(S_tr[:300])
+    c_p, i_p = d.forward(torch.from_numpy(S).to(DEV), 1e-3, max_iter=500, tol=1e-6)
+    c_f, info = sf.fista_batch(A_tr, S, 1e-3, max_iter=500, tol=1e-6, precision="fp32")
+    its = i_p.cpu().numpy().astype(np.int64)
+    assert its.min() < its.max()  # uneven convergence: slots were refilled
+    np.testing.assert_array_equal(its, info["iterations"].astype(np.int64))
+    np.testing.assert_array_equal(c_p.cpu().numpy(), c_f)
 
 
 def test_adjoint_matches_fp64_oracle(orc, problem):
