@@ -302,7 +302,7 @@ def tikhonov_bench(args):
                    "radial_order": 8, "solver": "tikhonov-l2 lambda_n=lambda_l=1e-2"},
         "roofline": {"bound": "mfma", "achieved": round(flops / kern_s / 1e12, 3), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": round(flops / kern_s / 1e12 / PEAK["fp32"], 4), "traffic": None,
-                     "kernel": "tikhonov_kernel<10>", "kernel_ms": round(kern_s * 1e3, 3),
+                     "kernel": "tikhonov_multi_kernel<10,2>", "kernel_ms": round(kern_s * 1e3, 3),
                      "flops_per_launch": flops, "algorithmic_bytes": nbytes,
                      "GB_s": round(nbytes / kern_s / 1e9, 1)},
     }

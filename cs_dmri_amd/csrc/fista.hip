@@ -473,9 +473,121 @@ __global__ void __launch_bounds__(256) tikhonov_kernel(const FistaArgs a, float*
     if (rss_l2) rss_l2[out_idx] = float(ss_res);
 }
 
+// Tikhonov with T tiles of 16 voxels per wave: every fragment of P (and of A
+// in the residual) is loaded once per wave and used for the T tiles, which
+// divides the fragment traffic from L2 by T (P and A are 174 KB each at order
+// 8, re-read by every wave).
+template <int MB, int T>
+__global__ void __launch_bounds__(256) tikhonov_multi_kernel(const FistaArgs a, float* rss_l2) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    int64_t v[T], out_idx[T];
+    bool valid[T];
+    const float* sig[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        v[t] = ((int64_t(blockIdx.x) * 4 + wave) * T + t) * 16 + j;
+        valid[t] = v[t] < a.n_vox;
+        out_idx[t] = valid[t] ? (a.vox ? a.vox[v[t]] : v[t]) : 0;
+        sig[t] = a.data + out_idx[t] * a.sig_stride;
+    }
+    // c = P s (the prologue contraction with P in place of c A^T)
+    floatx4 c[T][MB];
+    float sum[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        sum[t] = 0.f;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) c[t][mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+    for (int ch = 0; ch < a.NC; ++ch) {
+        float s4[T][4];
+        const int n0 = 16 * ch + 4 * h;
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int n = n0 + r;
+                s4[t][r] = (valid[t] && n < a.N) ? sig[t][n] : 0.f;
+                sum[t] += s4[t][r];
+            }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            const floatx4 af = a.ATfrag[(size_t(mb) * a.NC + ch) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) c[t][mb] = mfma4(af[r], s4[t][r], c[t][mb]);
+        }
+    }
+    float mean[T], rss[T], sst[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        mean[t] = lane_sum4(sum[t]) / float(a.N);
+        rss[t] = 0.f;
+        sst[t] = 0.f;
+    }
+    // residual |s - A c|^2 and the sum of squares about the mean
+    for (int nb = 0; nb < a.NC; ++nb) {
+        floatx4 acc[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < MB; ++kb) {
+            const floatx4 af = a.Afrag[(size_t(nb) * MB + kb) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[t] = mfma4(af[r], c[t][kb][r], acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int n = 16 * nb + 4 * h + r;
+                if (valid[t] && n < a.N) {
+                    const float sv = sig[t][n];
+                    const float e = sv - acc[t][r];
+                    rss[t] = fmaf(e, e, rss[t]);
+                    const float dm = sv - mean[t];
+                    sst[t] = fmaf(dm, dm, sst[t]);
+                }
+            }
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        rss[t] = lane_sum4(rss[t]);
+        sst[t] = lane_sum4(sst[t]);
+        if (!valid[t]) continue;
+        if (a.coef)
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = 16 * mb + 4 * h + r;
+                    if (k < a.K) a.coef[out_idx[t] * a.K + k] = c[t][mb][r];
+                }
+        if (h != 0) continue;
+        const double ss_res = double(rss[t]);
+        if (a.resid) a.resid[out_idx[t]] = float(sqrt(ss_res));
+        if (a.r2) a.r2[out_idx[t]] = float(sst[t] > 0.f ? 1.0 - ss_res / double(sst[t]) : 0.0);
+        if (a.iters) a.iters[out_idx[t]] = 1;
+        if (a.converged) a.converged[out_idx[t]] = 1;
+        if (a.kind) a.kind[out_idx[t]] = 2;
+        if (rss_l2) rss_l2[out_idx[t]] = float(ss_res);
+    }
+}
+
 template <int MB>
 void launch_tik(const FistaArgs& args, int64_t n_vox, float* rss_l2, hipStream_t s) {
     const int64_t tiles = (n_vox + 15) / 16;
+    if constexpr (MB >= 8 && MB <= 10) {
+        constexpr int T = 2;
+        hipLaunchKernelGGL((tikhonov_multi_kernel<MB, T>), dim3(unsigned((tiles + 4 * T - 1) / (4 * T))), dim3(256),
+                           0, s, args, rss_l2);
+        SF_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(tikhonov_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, args, rss_l2);
     SF_HIP(hipGetLastError());
 }

@@ -84,6 +84,10 @@ def test_forward_persistent_matches_plain_kernel(problem):
     assert its.min() < its.max()  # uneven convergence: slots were refilled
     np.testing.assert_array_equal(its, info["iterations"].astype(np.int64))
     np.testing.assert_array_equal(c_p.cpu().numpy(), c_f)
+    for n in (1, 17):  # fewer voxels than one wave's slots, and one more than a tile
+        c_n, i_n = d.forward(torch.from_numpy(S[:n].copy()).to(DEV), 1e-3, max_iter=500, tol=1e-6)
+        np.testing.assert_array_equal(c_n.cpu().numpy(), c_f[:n])
+        np.testing.assert_array_equal(i_n.cpu().numpy().astype(np.int64), its[:n])
 
 
 def test_adjoint_matches_fp64_oracle(orc, problem):
