@@ -230,36 +230,106 @@ __global__ void __launch_bounds__(256) cholesky_kernel(double* __restrict__ G, i
 // order, so the same bits) on the packed lower triangle held in LDS
 // (L[a][b] at a (a + 1) / 2 + b): the trailing updates are LDS traffic
 // instead of L2 round trips. Used when K (K + 1) / 2 doubles fit in LDS.
+// Blocked right-looking Cholesky on the packed lower triangle in LDS: panels
+// of kPanel columns. Wave 0 factors a panel on its own, in registers (no
+// workgroup barrier),
+// then every wave subtracts the panel's columns from the trailing triangle,
+// one column after another, so each entry receives exactly the updates of
+// the column-by-column algorithm in the same order (the same L bit for bit),
+// with two workgroup barriers per panel instead of three per column.
+constexpr int kPanel = 8;
+constexpr int kPanelRows = 3;  // rows per lane in the panel: K <= 192
+
+// Value of lane `src` (wave-uniform) of a double.
+__device__ __forceinline__ double lane_double(double x, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+
 __global__ void __launch_bounds__(1024) cholesky_lds_kernel(double* __restrict__ G, int K, int* __restrict__ status) {
     extern __shared__ double Lp[];
     __shared__ int bad;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    auto at = [&](int a, int b) -> double& { return Lp[a * (a + 1) / 2 + b]; };
     if (tid == 0) bad = 0;
-    for (int a = 0; a < K; ++a)
-        for (int b = tid; b <= a; b += nt) Lp[a * (a + 1) / 2 + b] = G[size_t(a) * K + b];
+    // one flat pass over G, so each thread's loads are independent
+    for (int e = tid; e < K * K; e += nt) {
+        const int a = e / K, b = e - a * K;
+        if (b <= a) at(a, b) = G[e];
+    }
     __syncthreads();
-    for (int j = 0; j < K; ++j) {
-        const double d = Lp[j * (j + 1) / 2 + j];
-        if (!(d > 0.0)) {
-            if (tid == 0) bad = j + 1;
-            break;
+    for (int j0 = 0; j0 < K; j0 += kPanel) {
+        const int j1 = min(j0 + kPanel, K);
+        if (wave == 0) {
+            // Panel columns j0..j1-1 in registers: lane t holds rows
+            // j0 + t + 64 q (q < kPanelRows); the pivot and the entries
+            // L[k][j] come from the lane that holds row k.
+            const int jb = j1 - j0;
+            double pv[kPanelRows][kPanel];
+#pragma unroll
+            for (int q = 0; q < kPanelRows; ++q) {
+                const int i = j0 + lane + 64 * q;
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c) pv[q][c] = (c < jb && i < K && j0 + c <= i) ? at(i, j0 + c) : 0.0;
+            }
+            bool fail = false;
+#pragma unroll
+            for (int c = 0; c < kPanel; ++c) {
+                if (c >= jb || fail) continue;
+                const int j = j0 + c;
+                const double d = lane_double(pv[0][c], c);
+                if (!(d > 0.0)) {
+                    if (lane == 0) bad = j + 1;
+                    fail = true;
+                    continue;
+                }
+                const double sj = sqrt(d);
+#pragma unroll
+                for (int q = 0; q < kPanelRows; ++q) {
+                    const int i = j0 + lane + 64 * q;
+                    if (i > j && i < K) pv[q][c] /= sj;
+                    else if (i == j) pv[q][c] = sj;
+                }
+                // the panel's later columns, rows k..K-1
+#pragma unroll
+                for (int k = c + 1; k < kPanel; ++k) {
+                    if (k >= jb) continue;
+                    const double lkj = lane_double(pv[0][c], k);
+#pragma unroll
+                    for (int q = 0; q < kPanelRows; ++q) {
+                        const int i = j0 + lane + 64 * q;
+                        if (i >= j0 + k && i < K) pv[q][k] -= pv[q][c] * lkj;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kPanelRows; ++q) {
+                const int i = j0 + lane + 64 * q;
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c)
+                    if (c < jb && i < K && j0 + c <= i) at(i, j0 + c) = pv[q][c];
+            }
         }
-        const double s = sqrt(d);
-        __syncthreads();  // every thread has read the pivot
-        for (int i = j + 1 + tid; i < K; i += nt) Lp[i * (i + 1) / 2 + j] /= s;
-        if (tid == 0) Lp[j * (j + 1) / 2 + j] = s;
         __syncthreads();
-        // trailing update: wave w takes rows j + 1 + w + 16 i, lanes the columns
-        const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-        for (int a = j + 1 + wave; a < K; a += nw) {
-            const double laj = Lp[a * (a + 1) / 2 + j];
-            for (int b = j + 1 + lane; b <= a; b += 64) Lp[a * (a + 1) / 2 + b] -= laj * Lp[b * (b + 1) / 2 + j];
+        if (bad) break;
+        // trailing triangle: rows a >= j1 (wave w takes a = j1 + w + nw i),
+        // columns j1..a (lanes), minus the panel's columns in order
+        for (int a = j1 + wave; a < K; a += nw) {
+            for (int b = j1 + lane; b <= a; b += 64) {
+                double v = at(a, b);
+                for (int j = j0; j < j1; ++j) v -= at(a, j) * at(b, j);
+                at(a, b) = v;
+            }
         }
         __syncthreads();
     }
     __syncthreads();
-    for (int a = 0; a < K; ++a)
-        for (int b = tid; b < K; b += nt) G[size_t(a) * K + b] = b > a ? 0.0 : Lp[a * (a + 1) / 2 + b];
+    for (int e = tid; e < K * K; e += nt) {
+        const int a = e / K, b = e - a * K;
+        G[e] = b > a ? 0.0 : Lp[a * (a + 1) / 2 + b];
+    }
     if (tid == 0) status[0] = bad;
 }
 
@@ -353,8 +423,10 @@ __global__ void __launch_bounds__(CB * 16) chol_solve_lds_kernel(const double* _
     const int tid = threadIdx.x, nt = CB * 16;
     const int c = tid >> 4, u = tid & 15;
     const int n0 = blockIdx.x * CB, n = n0 + c;
-    for (int a = 0; a < K; ++a)
-        for (int b = tid; b <= a; b += nt) Lp[a * (a + 1) / 2 + b] = Lf[size_t(a) * K + b];
+    for (int e = tid; e < K * K; e += nt) {  // flat, so each thread's loads are independent
+        const int a = e / K, b = e - a * K;
+        if (b <= a) Lp[a * (a + 1) / 2 + b] = Lf[e];
+    }
     for (int t = tid; t < K * CB; t += nt) {
         const int i = t / CB, cc = t % CB;
         Y[t] = n0 + cc < N ? A[size_t(n0 + cc) * K + i] : 0.0;
@@ -562,7 +634,7 @@ constexpr size_t kCholLds = 150 * 1024;  // LDS the packed-triangle kernels may 
 
 void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s) {
     const size_t lds = size_t(K) * (K + 1) / 2 * sizeof(double);
-    if (lds <= kCholLds) {
+    if (lds <= kCholLds && K <= kPanelRows * 64) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(cholesky_lds_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         hipLaunchKernelGGL(cholesky_lds_kernel, dim3(1), dim3(1024), lds, s, d_G, K, d_status);
