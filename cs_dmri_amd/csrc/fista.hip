@@ -1542,9 +1542,10 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
         return;
     }
     // persistent FP32 form: fixed alpha, no path, no residual maps, a work
-    // counter and a momentum table supplied
+    // counter and a momentum table supplied, and a voxel count the 32-bit
+    // work counter can hand out
     const bool persist = L.persist && L.path_n == 0 && L.ratio == 0.f && L.counter && L.mom && !L.resid &&
-                         !L.r2 && !L.bic;
+                         !L.r2 && !L.bic && L.n_vox < (int64_t(1) << 31);
     switch (L.Kp / 16) {
 #define SF_F32(MB)                                                  \
     case MB:                                                        \
