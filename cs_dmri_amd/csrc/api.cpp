@@ -293,7 +293,7 @@ void upload_momentum(DevBuf<float>& mom, uint32_t max_iter, hipStream_t s) {
 // L on an event while the right-hand-side kernel runs.
 void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStream_t s) {
     const int64_t V = L.n_vox;
-    const bool table = L.precision != SHOREFIT_PREC_FP32 || L.path_n > 0;
+    const bool table = L.precision != SHOREFIT_PREC_FP32 || L.path_n > 0 || L.persist;
     if (table) {
         // uploaded first, while the stream holds little work
         upload_momentum(w.mom, L.max_iter, s);
@@ -354,7 +354,7 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
         L.itspath = w.itspath.p;
         L.convpath = w.convpath.p;
     }
-    if (L.precision == SHOREFIT_PREC_FP32) return;
+    if (L.precision == SHOREFIT_PREC_FP32 && !(L.persist && L.path_n == 0)) return;
     w.counter.alloc(1);
     w.X.alloc(size_t(V) * d.Kp);
     w.its.alloc(V);
@@ -844,6 +844,7 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
         L.tol = float(sp.tol);
         L.nonneg = sp.non_negative;
         L.precision = precision;
+        L.persist = precision == SHOREFIT_PREC_FP32;  // FP32: the persistent form (no alpha path)
         L.refresh = std::max(1, sp.refresh_every);
         L.coef = coef;
         L.resid = res;

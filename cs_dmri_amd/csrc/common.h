@@ -178,7 +178,8 @@ struct FistaLaunch {
     float* alphabuf;        // [n_vox]
     void* event_mid;        // optional hipEvent_t recorded right after the solve kernel
     // FP32 only: persistent form (slots refilled from `counter`, momentum from
-    // `mom`) when alpha is fixed and no residual maps are asked for
+    // `mom`; with Xbuf / itsbuf / statusbuf / alphabuf the outputs go through
+    // the epilogue kernel, without them the coefficients are written directly)
     bool persist;
     // alpha path strategies (path_n > 0): point i of voxel v is the FISTA
     // solve at alpha_i = alpha_max(v) path_f[i], warm-started from point i-1;

@@ -608,10 +608,14 @@ void launch_rhs(const FistaLaunch& L, hipStream_t s) {
 // slot comes from the table of the FP64 recurrence (the values the plain
 // kernel computes), and every voxel's arithmetic is the plain kernel's, so
 // the coefficients and iteration counts are the same bit for bit.
-// Coefficients, iterations, convergence, kind and alpha are written as each
-// voxel stops (no residual maps: the caller uses the plain kernel for those).
+// Each voxel's outputs are written as it stops: the coefficients and
+// diagnostics directly, or (when residual maps are wanted) the iterate and
+// its status in work order for fista_epilogue_kernel, as the mixed kernel
+// does.
 template <int MB, int WAVES, bool M_IN_LDS>
-__global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const FistaArgs a, unsigned int* counter) {
+__global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const FistaArgs a, unsigned int* counter,
+                                                                       float* X, uint32_t* its_out,
+                                                                       uint8_t* status_out, float* alpha_out) {
     extern __shared__ floatx4 Ms[];
     const int lane = threadIdx.x & 63;
     const int h = lane >> 4, j = lane & 15;
@@ -634,8 +638,21 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
     bool active = false, zero_voxel = false;
     bool need = true;  // this slot (all four lanes of column j agree) wants a voxel
 
-    // outputs of the slot's voxel (src/solver/fista.rs:113-143 diagnostics)
+    // outputs of the slot's voxel (src/solver/fista.rs:113-143 diagnostics):
+    // with X, the iterate, iterations, status (bit 0 converged, bit 1 zero
+    // voxel) and alpha in work order, for fista_epilogue_kernel; without,
+    // the coefficients and diagnostics directly
     auto finish = [&](bool conv) {
+        if (X) {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) *reinterpret_cast<floatx4*>(X + v * a.Kp + 16 * mb + 4 * h) = x[mb];
+            if (h == 0) {
+                its_out[v] = its;
+                status_out[v] = uint8_t((conv ? 1 : 0) | (zero_voxel ? 2 : 0));
+                alpha_out[v] = zero_voxel ? 0.f : float(alpha_v);
+            }
+            return;
+        }
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
@@ -775,28 +792,6 @@ void launch_f32(const FistaArgs& args, int64_t n_vox, hipStream_t s) {
     SF_HIP(hipGetLastError());
 }
 
-
-template <int MB>
-void launch_f32_persist(const FistaArgs& args, unsigned int* counter, hipStream_t s) {
-    constexpr int WAVES = (MB >= 16) ? 4 : 8;
-    constexpr bool IN_LDS = (MB * MB * 64 * 16) <= 112 * 1024;
-    const size_t lds = IN_LDS ? size_t(MB) * MB * 64 * 16 : 0;
-    auto kern = fista_f32_persist_kernel<MB, WAVES, IN_LDS>;
-    if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  int(lds));
-    int dev = 0, cus = 0, per_cu = 0;
-    SF_HIP(hipGetDevice(&dev));
-    SF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 64 * WAVES,
-                                                        lds));
-    if (per_cu < 1) per_cu = 1;
-    const int64_t need = (args.n_vox + 16 * WAVES - 1) / (16 * WAVES);
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(need, int64_t(cus) * per_cu));
-    SF_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
-    hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(64 * WAVES), lds, s, args, counter);
-    SF_HIP(hipGetLastError());
-}
 
 // ---------------------------------------------------------------------------
 // Mixed-precision persistent form.
@@ -1395,6 +1390,36 @@ __global__ void __launch_bounds__(256) path_select_kernel(const FistaArgs a) {
 }
 
 template <int MB>
+void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t s) {
+    unsigned int* counter = L.counter;
+    constexpr int WAVES = (MB >= 16) ? 4 : 8;
+    constexpr bool IN_LDS = (MB * MB * 64 * 16) <= 112 * 1024;
+    const size_t lds = IN_LDS ? size_t(MB) * MB * 64 * 16 : 0;
+    auto kern = fista_f32_persist_kernel<MB, WAVES, IN_LDS>;
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(lds));
+    int dev = 0, cus = 0, per_cu = 0;
+    SF_HIP(hipGetDevice(&dev));
+    SF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 64 * WAVES,
+                                                        lds));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t need = (args.n_vox + 16 * WAVES - 1) / (16 * WAVES);
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(need, int64_t(cus) * per_cu));
+    SF_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(64 * WAVES), lds, s, args, counter, L.Xbuf, L.itsbuf,
+                       L.statusbuf, L.alphabuf);
+    SF_HIP(hipGetLastError());
+    if (L.event_mid) SF_HIP(hipEventRecord(static_cast<hipEvent_t>(L.event_mid), s));
+    if (!L.Xbuf) return;
+    const int64_t tiles = (args.n_vox + 15) / 16;
+    hipLaunchKernelGGL(fista_epilogue_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, args, L.Xbuf,
+                       L.itsbuf, L.statusbuf, L.alphabuf);
+    SF_HIP(hipGetLastError());
+}
+
+template <int MB>
 void launch_path_select(const FistaArgs& a, hipStream_t s) {
     const int64_t tiles = (a.n_vox + 15) / 16;
     hipLaunchKernelGGL(path_select_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, a);
@@ -1541,16 +1566,17 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
         }
         return;
     }
-    // persistent FP32 form: fixed alpha, no path, no residual maps, a work
-    // counter and a momentum table supplied, and a voxel count the 32-bit
-    // work counter can hand out
-    const bool persist = L.persist && L.path_n == 0 && L.ratio == 0.f && L.counter && L.mom && !L.resid &&
-                         !L.r2 && !L.bic && L.n_vox < (int64_t(1) << 31);
+    // persistent FP32 form: no alpha path, a work counter and a momentum
+    // table supplied, work-order buffers when residual maps are wanted, and
+    // a voxel count the 32-bit work counter can hand out
+    const bool maps = L.resid || L.r2 || L.bic;
+    const bool persist = L.persist && L.path_n == 0 && L.counter && L.mom && (L.Xbuf || !maps) &&
+                         L.n_vox < (int64_t(1) << 31);
     switch (L.Kp / 16) {
 #define SF_F32(MB)                                                  \
     case MB:                                                        \
         if (persist) {                                              \
-            launch_f32_persist<MB>(a, L.counter, s);                \
+            launch_f32_persist<MB>(a, L, s);                        \
             break;                                                  \
         }                                                           \
         launch_f32<MB>(a, L.n_vox, s);                              \
