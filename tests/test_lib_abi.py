@@ -65,6 +65,27 @@ def test_n_coeffs():
         n_coeffs(7)
 
 
+def test_graddev_variant_table():
+    """Every radial order 2-10 has variants (host-side table, no device
+    work): the padded coefficient count covers K in multiples of 16, the Gram
+    rows per chunk are a multiple of 4 of at most 64, and an index past the
+    order's variants leaves the outputs alone."""
+    from cs_dmri_amd import graddev_variants, n_coeffs
+    for order in (2, 4, 6, 8, 10):
+        vs = graddev_variants(order)
+        assert vs, order
+        K = n_coeffs(order)
+        for v in vs:
+            assert v["kp"] >= K and v["kp"] % 16 == 0 and v["kp"] - K < 16, (order, v)
+            assert v["rows"] % 4 == 0 and 4 <= v["rows"] <= 64, (order, v)
+            assert v["gram_waves"] >= 1 and v["fista_waves"] >= 1
+        assert len({v["kp"] for v in vs}) == 1, order
+        out = C.c_int32(-7)
+        n = L.lib().shorefit_graddev_variants(order, len(vs), C.byref(out), None, None, None)
+        assert n == len(vs) and out.value == -7
+    assert L.lib().shorefit_graddev_variants(12, 0, None, None, None, None) == 0
+
+
 def _series(shells):
     bvals = [0.0, 0.0]
     bvecs = [[0, 0, 1.0]] * 2
