@@ -354,8 +354,10 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
         L.itspath = w.itspath.p;
         L.convpath = w.convpath.p;
     }
-    if (L.precision == SHOREFIT_PREC_FP32 && !(L.persist && L.path_n == 0)) return;
+    if (L.precision == SHOREFIT_PREC_FP32 && !L.persist) return;
     w.counter.alloc(1);
+    L.counter = w.counter.p;
+    if (L.precision == SHOREFIT_PREC_FP32 && L.path_n > 0) return;  // the path points hold the results
     w.X.alloc(size_t(V) * d.Kp);
     w.its.alloc(V);
     w.status.alloc(V);

@@ -612,7 +612,7 @@ void launch_rhs(const FistaLaunch& L, hipStream_t s) {
 // diagnostics directly, or (when residual maps are wanted) the iterate and
 // its status in work order for fista_epilogue_kernel, as the mixed kernel
 // does.
-template <int MB, int WAVES, bool M_IN_LDS>
+template <int MB, int WAVES, bool M_IN_LDS, bool PATH>
 __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const FistaArgs a, unsigned int* counter,
                                                                        float* X, uint32_t* its_out,
                                                                        uint8_t* status_out, float* alpha_out) {
@@ -637,6 +637,8 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
     double alpha_v = a.alpha;
     bool active = false, zero_voxel = false;
     bool need = true;  // this slot (all four lanes of column j agree) wants a voxel
+    float bmax = 0.f;
+    int pi = 0;        // alpha-path point of the slot (PATH)
 
     // outputs of the slot's voxel (src/solver/fista.rs:113-143 diagnostics):
     // with X, the iterate, iterations, status (bit 0 converged, bit 1 zero
@@ -688,23 +690,29 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb)
                 if (load) b[mb] = *reinterpret_cast<const floatx4*>(a.B + v * a.Kp + 16 * mb + 4 * h);
-            // alpha-ratio strategy: thr_v = ratio * max|b| (src/solver/alpha.rs:229-235)
-            float bmax = 0.f;
+            // alpha-ratio strategy: thr_v = ratio * max|b| (src/solver/alpha.rs:229-235);
+            // path strategies: thr_i = step alpha_max f_i = max|b| f_i
+            float bm = 0.f;
 #pragma unroll
             for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) bmax = fmaxf(bmax, fabsf(b[mb][r]));
-            bmax = lane_max4(bmax);
+                for (int r = 0; r < 4; ++r) bm = fmaxf(bm, fabsf(b[mb][r]));
+            bm = lane_max4(bm);
             if (load) {
+                bmax = bm;
                 thr = a.thr;
                 alpha_v = a.alpha;
                 zero_voxel = false;
-                if (a.ratio > 0.f) {
+                pi = 0;
+                if constexpr (PATH) {
+                    zero_voxel = (bmax == 0.f);
+                    thr = bmax * a.path_f[0];
+                } else if (a.ratio > 0.f) {
                     zero_voxel = (bmax == 0.f);
                     thr = a.ratio * bmax;
                     alpha_v = double(bmax) / double(a.step) * double(a.ratio);
                 }
-                if (a.alpha_vox) {  // per-voxel alpha (fixed strategy): thr_v = alpha_v / L
+                if (!PATH && a.alpha_vox) {  // per-voxel alpha (fixed strategy): thr_v = alpha_v / L
                     const float av = a.alpha_vox[v];
                     thr = float(a.step64 * double(av));
                     alpha_v = double(av);
@@ -719,7 +727,14 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
                         y[mb][r] = x[mb][r];
                     }
                 its = 0;
-                if (zero_voxel || a.max_iter == 0) {
+                if (PATH && zero_voxel) {
+                    // every path point of an all-zero voxel: 0 iterations, status 3
+                    if (h == 0)
+                        for (int i = 0; i < a.path_n; ++i) {
+                            a.itspath[v * a.path_n + i] = 0;
+                            a.convpath[v * a.path_n + i] = 3;
+                        }
+                } else if (!PATH && (zero_voxel || a.max_iter == 0)) {
                     finish(zero_voxel);  // nothing to iterate: take another voxel next round
                 } else {
                     active = true;
@@ -764,14 +779,43 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
             }
         dd = lane_sum4(dd);
         xx = lane_sum4(xx);
+        bool next = false;
         if (active) {
             ++its;
             const float rel = sqrtf(dd) / fmaxf(sqrtf(xx), 1e-12f);
             const bool conv = rel < a.tol;
             if (conv || its >= a.max_iter) {
-                finish(conv);
-                active = false;
-                need = true;
+                if constexpr (PATH) {
+                    const int64_t pt = v * a.path_n + pi;
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb)
+                        *reinterpret_cast<floatx4*>(a.Xpath + pt * a.Kp + 16 * mb + 4 * h) = x[mb];
+                    if (h == 0) {
+                        a.itspath[pt] = its;
+                        a.convpath[pt] = conv ? 1 : 0;
+                    }
+                    if (pi + 1 < a.path_n) {
+                        // warm start of the next point: x0 = y0 = x, t = 1
+                        ++pi;
+                        thr = bmax * a.path_f[pi];
+                        its = 0;
+                        next = true;
+                    } else {
+                        active = false;
+                        need = true;
+                    }
+                } else {
+                    finish(conv);
+                    active = false;
+                    need = true;
+                }
+            }
+        }
+        if constexpr (PATH) {
+            if (__ballot(next) != 0ull) {
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+                    if (next) y[mb] = x[mb];
             }
         }
     }
@@ -1390,12 +1434,20 @@ __global__ void __launch_bounds__(256) path_select_kernel(const FistaArgs a) {
 }
 
 template <int MB>
+void launch_path_select(const FistaArgs& a, hipStream_t s) {
+    const int64_t tiles = (a.n_vox + 15) / 16;
+    hipLaunchKernelGGL(path_select_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, a);
+    SF_HIP(hipGetLastError());
+}
+
+template <int MB>
 void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t s) {
     unsigned int* counter = L.counter;
     constexpr int WAVES = (MB >= 16) ? 4 : 8;
     constexpr bool IN_LDS = (MB * MB * 64 * 16) <= 112 * 1024;
     const size_t lds = IN_LDS ? size_t(MB) * MB * 64 * 16 : 0;
-    auto kern = fista_f32_persist_kernel<MB, WAVES, IN_LDS>;
+    auto kern = L.path_n > 0 ? fista_f32_persist_kernel<MB, WAVES, IN_LDS, true>
+                             : fista_f32_persist_kernel<MB, WAVES, IN_LDS, false>;
     if (lds > 64 * 1024)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int(lds));
@@ -1412,17 +1464,14 @@ void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t
                        L.statusbuf, L.alphabuf);
     SF_HIP(hipGetLastError());
     if (L.event_mid) SF_HIP(hipEventRecord(static_cast<hipEvent_t>(L.event_mid), s));
+    if (L.path_n > 0) {
+        launch_path_select<MB>(args, s);  // the selection pass writes the outputs
+        return;
+    }
     if (!L.Xbuf) return;
     const int64_t tiles = (args.n_vox + 15) / 16;
     hipLaunchKernelGGL(fista_epilogue_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, args, L.Xbuf,
                        L.itsbuf, L.statusbuf, L.alphabuf);
-    SF_HIP(hipGetLastError());
-}
-
-template <int MB>
-void launch_path_select(const FistaArgs& a, hipStream_t s) {
-    const int64_t tiles = (a.n_vox + 15) / 16;
-    hipLaunchKernelGGL(path_select_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, a);
     SF_HIP(hipGetLastError());
 }
 
@@ -1566,11 +1615,11 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
         }
         return;
     }
-    // persistent FP32 form: no alpha path, a work counter and a momentum
-    // table supplied, work-order buffers when residual maps are wanted, and
-    // a voxel count the 32-bit work counter can hand out
+    // persistent FP32 form: a work counter and a momentum table supplied,
+    // work-order buffers when residual maps are wanted without an alpha path,
+    // and a voxel count the 32-bit work counter can hand out
     const bool maps = L.resid || L.r2 || L.bic;
-    const bool persist = L.persist && L.path_n == 0 && L.counter && L.mom && (L.Xbuf || !maps) &&
+    const bool persist = L.persist && L.counter && L.mom && (L.path_n > 0 || L.Xbuf || !maps) &&
                          L.n_vox < (int64_t(1) << 31);
     switch (L.Kp / 16) {
 #define SF_F32(MB)                                                  \

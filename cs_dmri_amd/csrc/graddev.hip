@@ -396,9 +396,11 @@ __global__ void __launch_bounds__(64 * WAVES, 2)
     if (tid < P) s_kap[tid] = B.kappa[tid];
     if (tid < Q) s_angc[tid] = B.ang_c[tid];
     // the padding columns K..KP-1 stay zero
-    for (int it = tid; it < RC * (KP - K); it += NTH) {
-        const int r = it / (KP - K);
-        s_A[r * LDA + K + (it - r * (KP - K))] = 0.0;
+    if constexpr (KP > K) {
+        for (int it = tid; it < RC * (KP - K); it += NTH) {
+            const int r = it / (KP - K);
+            s_A[r * LDA + K + (it - r * (KP - K))] = 0.0;
+        }
     }
     __syncthreads();
     if (w == 0) {
