@@ -301,7 +301,8 @@ def tikhonov_bench(args):
         "config": {"workload": "shore8-cartesian258-96x96x60-ellipsoid", "voxels": V, "n_grads": N, "n_coeffs": K,
                    "radial_order": 8, "solver": "tikhonov-l2 lambda_n=lambda_l=1e-2"},
         "roofline": {"bound": "mfma", "achieved": round(flops / kern_s / 1e12, 3), "peak": PEAK["fp32"],
-                     "unit": "TFLOP/s", "frac": round(flops / kern_s / 1e12 / PEAK["fp32"], 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(flops / kern_s / 1e12 / PEAK["fp32"], 4),
+                     "traffic": traffic_bytes("tikhonov"),
                      "kernel": "tikhonov_multi_kernel<10,2>", "kernel_ms": round(kern_s * 1e3, 3),
                      "flops_per_launch": flops, "algorithmic_bytes": nbytes,
                      "GB_s": round(nbytes / kern_s / 1e9, 1)},
