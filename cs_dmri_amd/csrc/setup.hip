@@ -413,41 +413,69 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
 // P = (L L^T)^-1 A^T for CB columns per workgroup: L (packed lower triangle)
 // and the workgroup's CB columns in LDS, 16 threads per column summing
 // strided partial dot products, combined by a butterfly.
+// P = L^-T L^-1 A^T, 16 columns of P per workgroup. The 16 lanes of a column
+// (all in one wave) keep its vector in registers, lane u the entries
+// u + 16 r; a substitution step divides the solved entry on its lane, hands it
+// to the other 15 with a 16-lane shuffle and lets each lane update its own
+// entries (L y = a: the entries below; L^T p = y: the entries above), so no
+// step waits on a workgroup barrier.
+constexpr int kSolveRows = 12;  // entries per lane: K <= 192
+
 template <int CB>
 __global__ void __launch_bounds__(CB * 16) chol_solve_lds_kernel(const double* __restrict__ Lf, int K,
                                                                  const double* __restrict__ A, int N,
                                                                  double* __restrict__ P) {
-    extern __shared__ double sm[];
-    double* Lp = sm;                              // K (K + 1) / 2
-    double* Y = sm + size_t(K) * (K + 1) / 2;     // [K][CB]
+    extern __shared__ double Lp[];  // K (K + 1) / 2
     const int tid = threadIdx.x, nt = CB * 16;
     const int c = tid >> 4, u = tid & 15;
-    const int n0 = blockIdx.x * CB, n = n0 + c;
+    const int n = blockIdx.x * CB + c;
     for (int e = tid; e < K * K; e += nt) {  // flat, so each thread's loads are independent
         const int a = e / K, b = e - a * K;
         if (b <= a) Lp[a * (a + 1) / 2 + b] = Lf[e];
     }
-    for (int t = tid; t < K * CB; t += nt) {
-        const int i = t / CB, cc = t % CB;
-        Y[t] = n0 + cc < N ? A[size_t(n0 + cc) * K + i] : 0.0;
+    double yv[kSolveRows];
+#pragma unroll
+    for (int r = 0; r < kSolveRows; ++r) {
+        const int k = u + 16 * r;
+        yv[r] = (n < N && k < K) ? A[size_t(n) * K + k] : 0.0;
     }
     __syncthreads();
-    for (int i = 0; i < K; ++i) {  // L y = a
-        double part = 0.0;
-        for (int k = u; k < i; k += 16) part += Lp[i * (i + 1) / 2 + k] * Y[k * CB + c];
-        for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
-        if (u == 0) Y[i * CB + c] = (Y[i * CB + c] - part) / Lp[i * (i + 1) / 2 + i];
-        __syncthreads();
+    // L y = a
+#pragma unroll
+    for (int r = 0; r < kSolveRows; ++r) {
+        for (int uu = 0; uu < 16; ++uu) {
+            const int i = 16 * r + uu;
+            if (i >= K) break;
+            const double yi = __shfl(yv[r], uu, 16) / Lp[i * (i + 1) / 2 + i];
+            if (u == uu) yv[r] = yi;
+#pragma unroll
+            for (int rr = r; rr < kSolveRows; ++rr) {
+                const int k = u + 16 * rr;
+                if (k > i && k < K) yv[rr] -= Lp[k * (k + 1) / 2 + i] * yi;
+            }
+        }
     }
-    for (int i = K - 1; i >= 0; --i) {  // L^T p = y
-        double part = 0.0;
-        for (int k = i + 1 + u; k < K; k += 16) part += Lp[k * (k + 1) / 2 + i] * Y[k * CB + c];
-        for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
-        if (u == 0) Y[i * CB + c] = (Y[i * CB + c] - part) / Lp[i * (i + 1) / 2 + i];
-        __syncthreads();
+    // L^T p = y
+#pragma unroll
+    for (int r = kSolveRows - 1; r >= 0; --r) {
+        for (int uu = 15; uu >= 0; --uu) {
+            const int i = 16 * r + uu;
+            if (i >= K) continue;
+            const double pi = __shfl(yv[r], uu, 16) / Lp[i * (i + 1) / 2 + i];
+            if (u == uu) yv[r] = pi;
+#pragma unroll
+            for (int rr = 0; rr <= r; ++rr) {
+                const int k = u + 16 * rr;
+                if (k < i) yv[rr] -= Lp[i * (i + 1) / 2 + k] * pi;
+            }
+        }
     }
     if (n < N)
-        for (int i = u; i < K; i += 16) P[size_t(i) * N + n] = Y[i * CB + c];
+#pragma unroll
+        for (int r = 0; r < kSolveRows; ++r) {
+            const int k = u + 16 * r;
+            if (k < K) P[size_t(k) * N + n] = yv[r];
+        }
 }
 
 __global__ void chol_solve_columns_kernel(const double* __restrict__ Lf, int K, const double* __restrict__ A, int N,
@@ -668,8 +696,8 @@ void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, u
 
 void launch_chol_solve_columns(const double* d_L, int K, const double* d_A, int N, double* d_P, hipStream_t s) {
     constexpr int CB = 16;  // columns per workgroup
-    const size_t lds = (size_t(K) * (K + 1) / 2 + size_t(K) * CB) * sizeof(double);
-    if (lds <= kCholLds) {
+    const size_t lds = size_t(K) * (K + 1) / 2 * sizeof(double);
+    if (lds <= kCholLds && K <= 16 * kSolveRows) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(chol_solve_lds_kernel<CB>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         hipLaunchKernelGGL(chol_solve_lds_kernel<CB>, dim3(unsigned((N + CB - 1) / CB)), dim3(CB * 16), lds, s, d_L,
