@@ -123,40 +123,57 @@ __device__ double block_sum(double v, double* red) {
 // polished by power sweeps on G until it stops moving (its error is quadratic
 // in the eigenvector error, so it reaches the FP64 floor; it never exceeds
 // l1). This replaces the reference's full SVD (src/solver/fista.rs:45-51).
+constexpr int kMvSplit = 7;  // threads per row of a matrix-vector product (1024 threads: 146 rows a pass)
+
 __global__ void __launch_bounds__(1024) power_iteration_kernel(const double* __restrict__ P,
                                                                const double* __restrict__ G, int K,
                                                                double* __restrict__ out) {
     extern __shared__ double sm[];
     double* v = sm;
     double* w = sm + K;
+    double* parts = sm + 2 * K;  // [kMvSplit][rows a pass]
     __shared__ double red[16];
-    for (int i = threadIdx.x; i < K; i += blockDim.x) v[i] = 1.0 + 1e-3 * i;
+    const int t = threadIdx.x, nrow = int(blockDim.x) / kMvSplit;
+    const int chunk = (K + kMvSplit - 1) / kMvSplit;
+    // w = M v for symmetric M (row p read as column p): thread (part, p)
+    // sums the products of columns part * chunk .. over its row, consecutive
+    // threads on consecutive rows; the kMvSplit partial sums of a row are
+    // then added in part order
+    auto matvec = [&](const double* __restrict__ M) {
+        for (int p0 = 0; p0 < K; p0 += nrow) {
+            const int part = t / nrow, p = p0 + t % nrow;
+            if (part < kMvSplit && p < K) {
+                const int q0 = part * chunk, q1 = min(K, q0 + chunk);
+                double acc = 0.0;
+#pragma unroll 8
+                for (int q = q0; q < q1; ++q) acc += M[size_t(q) * K + p] * v[q];
+                parts[part * nrow + (p - p0)] = acc;
+            }
+            __syncthreads();
+            if (t < nrow && p0 + t < K) {
+                double acc = 0.0;
+                for (int i = 0; i < kMvSplit; ++i) acc += parts[i * nrow + t];
+                w[p0 + t] = acc;
+            }
+            __syncthreads();
+        }
+    };
+    for (int i = t; i < K; i += blockDim.x) v[i] = 1.0 + 1e-3 * i;
     __syncthreads();
     // phase 1: eigenvector of P
     for (int it = 0; it < 400; ++it) {
+        matvec(P);
         double nrm = 0.0, diff = 0.0;
-        for (int p = threadIdx.x; p < K; p += blockDim.x) {
-            // P is symmetric bit for bit (a product of symmetric matrices
-            // summed in the same order for (p, q) and (q, p)), so row p is
-            // read as column p: consecutive threads read consecutive entries
-            double acc = 0.0;
-#pragma unroll 8
-            for (int q = 0; q < K; ++q) acc += P[size_t(q) * K + p] * v[q];
-            w[p] = acc;
-            nrm += acc * acc;
-        }
+        for (int p = t; p < K; p += blockDim.x) nrm += w[p] * w[p];
         const double ww = block_sum(nrm, red);
         if (!(ww > 0.0)) break;
         const double inv = 1.0 / sqrt(ww);
-        for (int p = threadIdx.x; p < K; p += blockDim.x) {
-            const double nv = w[p] * inv;
-            diff = fmax(diff, fabs(nv - v[p]));
-        }
+        for (int p = t; p < K; p += blockDim.x) diff = fmax(diff, fabs(w[p] * inv - v[p]));
         __syncthreads();
-        for (int p = threadIdx.x; p < K; p += blockDim.x) v[p] = w[p] * inv;
+        for (int p = t; p < K; p += blockDim.x) v[p] = w[p] * inv;
         for (int o = 32; o > 0; o >>= 1) diff = fmax(diff, __shfl_xor(diff, o));
         __syncthreads();
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = diff;
+        if ((t & 63) == 0) red[t >> 6] = diff;
         __syncthreads();
         double dmax = 0.0;
         for (int i = 0; i < int(blockDim.x >> 6); ++i) dmax = fmax(dmax, red[i]);
@@ -167,14 +184,11 @@ __global__ void __launch_bounds__(1024) power_iteration_kernel(const double* __r
     double lam = 0.0;
     int stable = 0;
     for (int it = 0; it < 200 && stable < 2; ++it) {
+        matvec(G);
         double part = 0.0, nrm = 0.0, vpart = 0.0;
-        for (int p = threadIdx.x; p < K; p += blockDim.x) {
-            double acc = 0.0;
-#pragma unroll 8
-            for (int q = 0; q < K; ++q) acc += G[size_t(q) * K + p] * v[q];  // G symmetric, as P
-            w[p] = acc;
-            part += acc * v[p];
-            nrm += acc * acc;
+        for (int p = t; p < K; p += blockDim.x) {
+            part += w[p] * v[p];
+            nrm += w[p] * w[p];
             vpart += v[p] * v[p];
         }
         const double vv = block_sum(vpart, red);
@@ -185,11 +199,11 @@ __global__ void __launch_bounds__(1024) power_iteration_kernel(const double* __r
         lam = fmax(lam, lam_new);
         const double inv = ww > 0.0 ? 1.0 / sqrt(ww) : 0.0;
         __syncthreads();
-        for (int p = threadIdx.x; p < K; p += blockDim.x) v[p] = w[p] * inv;
+        for (int p = t; p < K; p += blockDim.x) v[p] = w[p] * inv;
         __syncthreads();
         if (!(ww > 0.0)) break;
     }
-    if (threadIdx.x == 0) out[0] = lam;
+    if (t == 0) out[0] = lam;
 }
 
 // Right-looking Cholesky in one workgroup (K <= a few hundred).
@@ -636,8 +650,8 @@ void launch_power_iteration(const double* d_G, int K, double* d_lambda, hipStrea
         hipLaunchKernelGGL(square_scaled_kernel, dim3(blocks_for(K, 64), K), dim3(64), 0, s, src, K, dst);
         src = dst;
     }
-    hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(1024), size_t(2 * K) * sizeof(double), s, src, d_G, K,
-                       d_lambda);
+    hipLaunchKernelGGL(power_iteration_kernel, dim3(1), dim3(1024),
+                       size_t(2 * K + kMvSplit * (1024 / kMvSplit)) * sizeof(double), s, src, d_G, K, d_lambda);
     SF_HIP(hipGetLastError());
     sync_if_unscoped(s);  // p0 / p1 are released at scope exit
 }
