@@ -481,7 +481,7 @@ std::map<std::pair<uint32_t, int64_t>, int>& tuned_variants() {
 }
 
 // The requested variant, else the tuned one for (order, N), else the first
-// (the default) of the order.
+// (the default) of the order whose Gram kernel fits its LDS for N gradients.
 const GdVariant& pick_variant(uint32_t order, int64_t N, int32_t requested) {
     const GdVariant* vs = nullptr;
     const int nv = gd_variants(order, &vs);
@@ -496,7 +496,10 @@ const GdVariant& pick_variant(uint32_t order, int64_t N, int32_t requested) {
     }
     std::lock_guard<std::mutex> lk(tuned_mutex());
     const auto it = tuned_variants().find({order, N});
-    return it != tuned_variants().end() ? vs[it->second] : vs[0];
+    if (it != tuned_variants().end()) return vs[it->second];
+    for (int i = 0; i < nv; ++i)
+        if (gd_gram_fits(vs[i], int(N))) return vs[i];
+    return vs[0];  // none fits: the launch reports it
 }
 
 // Identity voxels (max |T - I| < 1e-6) of the voxel list into id, the others
@@ -1319,8 +1322,13 @@ int shorefit_graddev_tune(const ShorefitGradients* grads, const ShorefitVolume* 
         SF_HIP(hipEventCreate(&e0));
         SF_HIP(hipEventCreate(&e1));
         double best_ms = 0.0;
+        const GdVariant& dflt = pick_variant(sp.radial_order, N, -1);
         for (int i = -1; i < nv; ++i) {  // i = -1: warm-up with the default
-            const GdVariant& v = vs[std::max(i, 0)];
+            const GdVariant& v = i < 0 ? dflt : vs[i];
+            if (!gd_gram_fits(v, int(N))) {  // too many gradients for this chunk's LDS
+                if (ms_out && i >= 0) ms_out[i] = -1.0;
+                continue;
+            }
             SF_HIP(hipEventRecord(e0, s));
             gd_run(S, v, sp, data, N, field, d_dv.p, n, o, nullptr, s);
             SF_HIP(hipEventRecord(e1, s));
@@ -1336,6 +1344,8 @@ int shorefit_graddev_tune(const ShorefitGradients* grads, const ShorefitVolume* 
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        if (best < 0)
+            throw Error{SHOREFIT_E_UNSUPPORTED, "gradient-deviation fit: too many gradients for LDS"};
         std::lock_guard<std::mutex> lk(tuned_mutex());
         tuned_variants()[{sp.radial_order, N}] = best;
     });
@@ -1350,7 +1360,7 @@ int shorefit_graddev_gram(const ShorefitGradients* grads, const ShorefitVolume* 
             throw Error{SHOREFIT_E_INVALID, "null argument"};
         if (vol->on_device) throw Error{SHOREFIT_E_INVALID, "shorefit_graddev_gram takes host memory"};
         check_dims(vol->nx, vol->ny, vol->nz);
-        const GdVariant& v = pick_variant(radial_order, grads->n_grads, 0);
+        const GdVariant& v = pick_variant(radial_order, grads->n_grads, -1);
         const int64_t N = grads->n_grads;
         if (N <= 0 || !grads->bvals || !grads->bvecs) throw Error{SHOREFIT_E_INVALID, "empty gradient table"};
         const int64_t nvox = vol->nx * vol->ny * vol->nz;

@@ -303,6 +303,8 @@ struct GdVariant {
     int order, kp, rc, gram_waves, fista_waves;
 };
 int gd_variants(uint32_t order, const GdVariant** first);
+// Whether variant v's Gram kernel fits its LDS for N gradients.
+bool gd_gram_fits(const GdVariant& v, int N);
 void gd_launch_gram(const GdVariant& v, const GdChunk& c, hipStream_t s);
 void gd_launch_fista(const GdVariant& v, const GdFistaArgs& a, int max_blocks, hipStream_t s);
 void gd_launch_finish(const GdSlotOut* so, const double* sstat, const int64_t* vox, int64_t n, int N,

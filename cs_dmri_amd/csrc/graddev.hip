@@ -853,6 +853,12 @@ static const GdVariant kVariants[] = {
     {10, 256, 32, 8, 8},
 };
 
+bool gd_gram_fits(const GdVariant& v, int N) {
+    const int R = v.order;
+    const size_t lds = sizeof(double) * (size_t(n_pairs_c(R) + n_ang_c(R)) + size_t(6) * N + size_t(v.rc) * (v.kp + 2));
+    return lds <= 160 * 1024;
+}
+
 int gd_variants(uint32_t order, const GdVariant** out) {
     int first = -1, count = 0;
     for (int i = 0; i < int(sizeof(kVariants) / sizeof(kVariants[0])); ++i)

@@ -176,7 +176,9 @@ int shorefit_graddev_variants(uint32_t radial_order, int32_t i, int32_t* kp, int
  * every variant of spec->radial_order, stores the fastest for (radial order,
  * N) in this process (used by shorefit_fit_graddev with variant -1) and
  * writes each variant's device time in ms to ms_out (n_variants entries, may
- * be NULL). Returns the chosen variant's index, or a negative status. */
+ * be NULL; -1 for a variant whose Gram kernel cannot hold the table's
+ * gradients in LDS). Returns the chosen variant's index, or a negative
+ * status. */
 int shorefit_graddev_tune(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitGradDev* gd,
                           const ShorefitSpec* spec, int64_t max_voxels, double* ms_out, void* stream);
 

@@ -156,6 +156,29 @@ def test_variants_and_chunks_agree(order):
     np.testing.assert_array_equal(chunked["iterations"], base["iterations"])
 
 
+def test_many_gradients_fall_back_to_a_smaller_chunk(orc):
+    """774 gradients at order 10: the default Gram chunk (64 design rows)
+    does not fit LDS, so the default falls back to the first variant that
+    does; G_v still matches the oracle and the fit matches the explicitly
+    chosen 16-row variant."""
+    gt, data, field = _volume((2, 1, 1))
+    bvals, bvecs = np.tile(gt.bvals, 3), np.tile(gt.bvecs, (3, 1))
+    data3 = np.ascontiguousarray(np.tile(data, (1, 1, 1, 3)))
+    G, _, _ = sf.graddev_gram(data3, bvals, bvecs, field, np.array([0, 1]), radial_order=10, **_kw(gt))
+    T = field.reshape(-1, 3, 3).astype(np.float64)
+    for v in (0, 1):
+        be, d = orc.effective_table(T[v], np.eye(3), bvals, bvecs, bvecs)
+        A = orc.design_matrix(10, 700.0, be, d, gt.big_delta, gt.small_delta)
+        Gr = A.T @ A
+        assert np.abs(G[v] - Gr).max() <= 2e-7 * np.abs(Gr).max()
+    kw = dict(radial_order=10, alpha_mode="fixed", alpha=1e-3, max_iter=30, tol=0.0, graddev=field, **_kw(gt))
+    m = np.ones((2, 1, 1), bool)
+    a = sf.shore_fit(data3, bvals, bvecs, m, **kw)
+    i16 = [v["rows"] for v in sf.graddev_variants(10)].index(16)
+    b = sf.shore_fit(data3, bvals, bvecs, m, graddev_variant=i16, **kw)
+    np.testing.assert_array_equal(a["coefficients"], b["coefficients"])
+
+
 def test_autotune_picks_a_variant():
     gt, data, field = _volume((4, 4, 2), seed=4)
     res = sf.graddev_tune(data, gt.bvals, gt.bvecs, np.ones(data.shape[:3], bool), field, radial_order=8,
