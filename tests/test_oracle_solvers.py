@@ -166,6 +166,26 @@ def test_tikhonov_normal_equations_on_shore_design(orc):
         assert r == pytest.approx(np.linalg.norm(y - A @ c), rel=1e-10)
 
 
+def test_tikhonov_normal_equations_order8_cartesian(orc):
+    # the whole-brain L2 leg: order 8 (K = 145) on 1 b0 + 257 Cartesian
+    # q-points, lambda_n = lambda_l = 1e-2; the reference's 1e-10 bound
+    # (tests/optimization_kkt.rs:164-190) and scipy's cho_solve
+    from cs_dmri_amd import workloads as W
+    gt = W.cartesian_table(257, seed=3)
+    A = orc.design_matrix(8, 700.0, gt.bvals, gt.bvecs, gt.big_delta, gt.small_delta)
+    dn, dl = orc.regularization(8)
+    assert A.shape == (258, 145)
+    S = W.multi_tensor_signal(gt, 8, W.S0 / 20.0, seed=2).astype(np.float64)
+    C, R = orc.tikhonov(A, S, dn, dl, 1e-2, 1e-2)
+    Greg = A.T @ A + 1e-2 * np.diag(dn + dl)
+    fac = sla.cho_factor(Greg)
+    for s, c, r in zip(S, C, R):
+        rhs = A.T @ s
+        assert np.linalg.norm(Greg @ c - rhs) / np.linalg.norm(rhs) < 1e-10
+        np.testing.assert_allclose(c, sla.cho_solve(fac, rhs), rtol=1e-6, atol=1e-9 * np.abs(c).max())
+        assert r == pytest.approx(np.linalg.norm(s - A @ c), rel=1e-10)
+
+
 def test_tikhonov_limits_and_not_pd(orc):
     A = make_design(200, 30, 0xD1)
     y = make_noisy_signal(A, make_planted_beta(30, 6, True, 0xD2), 0.05, 0xD3)
