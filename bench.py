@@ -46,6 +46,48 @@ def traffic_bytes(precision: str):
     return round(t["fetch_kib"] * t["fetch_scale"] * 1024 + t["write_kib"] * t["write_scale"] * 1024)
 
 
+def cpu_threads():
+    """(threads for the CPU baseline, CPUs this process may run on, host CPU
+    count). The threads are the CPUs in this process's affinity mask, capped
+    by OMP_NUM_THREADS when the environment sets it (the CPU share a shared
+    GPU host allots to one GPU's job)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(avail, share) if share > 0 else avail), avail, os.cpu_count() or avail
+
+
+def cpu_fit_rate(orc, S, A, N, K, *, threads, **kw):
+    """FP64 oracle fits of the rows of S (one voxel per task on `threads`
+    threads): (fits/s, wall s, result dict)."""
+    t0 = time.perf_counter()
+    ref = orc.fit_volume(S.reshape(len(S), 1, 1, N), None, A, n_threads=threads, **kw)
+    wall = time.perf_counter() - t0
+    return len(S) / wall, wall, ref
+
+
+def err_stats(c, ref, its=None, its_ref=None) -> dict:
+    """Per-voxel relative coefficient error of c against ref (rows)."""
+    c, ref = np.asarray(c, np.float64), np.asarray(ref, np.float64)
+    e = np.linalg.norm(c - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
+    out = {"sample": len(e), "coef_rel_err_median": float(np.median(e)), "coef_rel_err_p99": float(np.quantile(e, 0.99)),
+           "coef_rel_err_max": float(e.max()), "frac_le_1e-4": float(np.mean(e <= 1e-4))}
+    if its is not None:
+        out["iterations_equal_frac"] = float(np.mean(np.asarray(its) == np.asarray(its_ref)))
+    return out
+
+
+def refresh_used(args, Kp: int) -> int:
+    """The refresh period the library applies: --refresh, else its default
+    for the arithmetic (FP16 with split increments, padded K <= 160: 40;
+    otherwise 10)."""
+    if args.refresh:
+        return args.refresh
+    return 40 if (args.precision == "fp16" and Kp <= 160) else 10
+
+
 def d_Kp(K: int, precision: str) -> int:
     """Padded coefficient count the kernels use (multiples of 16 / 32)."""
     sizes = [16, 32, 48, 64, 80, 96, 128, 160, 192, 256] if precision == "fp32" else [32, 64, 96, 128, 160, 256]
@@ -58,8 +100,13 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--precision", default="fp16", choices=["fp32", "fp16", "bf16"])
-    p.add_argument("--refresh", type=int, default=10)
+    p.add_argument("--refresh", type=int, default=None,
+                   help="exact-refresh period of the mixed kernels (default: the library's for the arithmetic)")
     p.add_argument("--cpu-sample", type=int, default=32768)
+    p.add_argument("--cpu-single-sample", type=int, default=1024,
+                   help="voxels of the single-thread CPU-baseline run")
+    p.add_argument("--equal-iter-sample", type=int, default=4096,
+                   help="voxels of the equal-iteration (tol = 0) accuracy check")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-postfit", action="store_true", help="skip the post-fit map timings")
     p.add_argument("--precision-report", action="store_true",
@@ -105,7 +152,7 @@ def precision_report():
     A = orc.design_matrix(6, 700.0, g.bvals, g.bvecs, g.big_delta, g.small_delta)
     t0 = time.perf_counter()
     ref = orc.fit_volume(wl.data, wl.mask, A, alpha_mode="fixed", alpha=1e-3, max_iter=500, tol=1e-6,
-                         n_threads=min(os.cpu_count() or 1, 16))["coefficients"]
+                         n_threads=cpu_threads()[0])["coefficients"]
     rep = report(wl.data, wl.mask, g, ref)
     rep["workload"] = "shore6-cartesian101-16x16x8 (2048 voxels), fista-l1 alpha=1e-3, max_iter 500, tol 1e-6"
     rep["fp64_reference_s"] = round(time.perf_counter() - t0, 2)
@@ -314,7 +361,7 @@ def tikhonov_bench(args):
         dn, dl = orc.regularization(wl.radial_order)
         S = wl.data[wl.mask]
         pick = np.arange(V)  # a Tikhonov fit is one solve per voxel: the whole mask takes about a second
-        threads = min(os.cpu_count() or 1, 16)
+        threads, _, host = cpu_threads()
         t0 = time.perf_counter()
         ref = orc.fit_volume(S[pick].reshape(len(pick), 1, 1, N), None, A, regularization="l2", dn=dn, dl=dl,
                              lambda_n=1e-2, lambda_l=1e-2, n_threads=threads)
@@ -325,7 +372,7 @@ def tikhonov_bench(args):
         rg = d_res.cpu().numpy()[wl.mask][pick].astype(np.float64)
         rr = ref["residual_l2"].reshape(len(pick)).astype(np.float64)
         result["cpu_baseline"] = {"value": round(len(pick) / cpu_wall, 2), "unit": "voxel-fits/s", "cores": threads,
-                                  "kind": "port",
+                                  "host_cores": host, "kind": "port",
                                   "sample": f"all {len(pick)} masked voxels, same fit settings, "
                                             f"FP64 oracle (C++), {cpu_wall:.2f} s wall"}
         result["accuracy_vs_fp64"] = {"sample": len(pick), "coef_rel_err_median": float(np.median(err)),
@@ -438,7 +485,7 @@ def graddev_bench(args):
             orc = load()
             sub = wl.data.reshape(-1, N)[flat_idx].reshape(len(pick), 1, 1, N)
             fs = field.reshape(-1, 3, 3)[flat_idx].reshape(len(pick), 1, 1, 3, 3)
-            threads = min(os.cpu_count() or 1, 16)
+            threads, _, host = cpu_threads()
             t0 = time.perf_counter()
             ref = orc.fit_graddev(sub, None, fs, g.bvals, g.bvecs, big_delta=g.big_delta, small_delta=g.small_delta,
                                   radial_order=order, max_iter=args.max_iter, tol=1e-6, n_threads=threads)
@@ -447,7 +494,7 @@ def graddev_bench(args):
             cg = d_coef.cpu().numpy()[wl.mask][pick].astype(np.float64)
             err = np.linalg.norm(cg - cr, axis=1) / np.maximum(np.linalg.norm(cr, axis=1), 1e-30)
             result["cpu_baseline"] = {"value": round(len(pick) / cpu_wall, 2), "unit": "voxel-fits/s",
-                                      "cores": threads, "kind": "port",
+                                      "cores": threads, "host_cores": host, "kind": "port",
                                       "sample": f"{len(pick)} voxels evenly spaced over the mask, same fit settings "
                                                 f"(per-voxel design, Jacobi Lipschitz), FP64 oracle (C++), "
                                                 f"{cpu_wall:.1f} s wall"}
@@ -572,7 +619,7 @@ def adjoint_bench(args):
         ns = 4096
         cs = np.linspace(0, V - 1, ns).astype(np.int64)
         sub = np.ascontiguousarray(sig[cs][:, tr], np.float32).reshape(ns, 1, 1, len(tr))
-        threads = min(os.cpu_count() or 1, 16)
+        threads, _, host = cpu_threads()
         t1 = time.perf_counter()
         ref = orc.fit_volume(sub, None, A_tr, alpha_mode="fixed", alpha=1e-3, max_iter=args.max_iter, tol=1e-6,
                              n_threads=threads)
@@ -581,7 +628,7 @@ def adjoint_bench(args):
         orc.lasso_adjoint(A_tr, rc, rg)
         cpu_s = time.perf_counter() - t1
         result["cpu_baseline"] = {"value": round(ns / cpu_s, 1), "unit": "voxel-gradients/s", "cores": threads,
-                                  "kind": "port",
+                                  "host_cores": host, "kind": "port",
                                   "sample": f"{ns} voxels evenly spaced: FP64 oracle fit ({threads} threads) and "
                                             f"FP64 adjoint (1 thread) of the held-out MSE, {cpu_s:.1f} s"}
     # --- held-out alpha: hypergradient descent vs the 20-point grid -----------
@@ -688,10 +735,13 @@ def main():
     if args.precision == "fp32":
         executed = its_sum * 2.0 * Kp * Kp
     else:
-        # increments on FP16/BF16 MFMA; refresh every k iterations: three FP16
-        # passes (fp16) or one FP32 MFMA pass (bf16)
-        k = args.refresh
-        executed = its_sum * 2.0 * Kp * Kp * ((k - 1) / k + (3.0 if args.precision == "fp16" else 1.0) / k)
+        # increments on FP16/BF16 MFMA, two passes (both halves of the split
+        # operator) at padded K <= 160; refresh every k iterations: one FP32
+        # MFMA pass, or three FP16 passes (fp16 at padded K > 160)
+        k = refresh_used(args, Kp)
+        split = Kp <= 160
+        refresh_passes = 3.0 if (args.precision == "fp16" and not split) else 1.0
+        executed = its_sum * 2.0 * Kp * Kp * ((2.0 if split else 1.0) * (k - 1) / k + refresh_passes / k)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": traffic_bytes(args.precision),
                 "kernel": KERNEL[args.precision], "kernel_ms": round(kern_s * 1e3, 3),
@@ -711,49 +761,49 @@ def main():
                    "n_coeffs": K, "radial_order": 8, "solver": ("fista-l1 fixed alpha=1e-3" if args.alpha_mode == "fixed"
                                                              else f"fista-l1 {args.alpha_mode}"),
                    "max_iter": args.max_iter, "tol": 1e-6, "precision": args.precision,
-                   "refresh_every": args.refresh if args.precision != "fp32" else None},
+                   "refresh_every": refresh_used(args, Kp) if args.precision != "fp32" else None},
         "iterations": {"mean": float(its.mean()), "p50": float(np.median(its)), "p99": float(np.quantile(its, 0.99)),
                        "max": int(its.max()), "at_cap": float(np.mean(its >= args.max_iter))},
         "roofline": roofline,
     }
 
-    # --- CPU baseline leg: the FP64 oracle on a bounded sample ------------------
+    # --- CPU baseline leg and accuracy: the FP64 oracle on bounded samples -----
     if not args.no_cpu and args.alpha_mode == "fixed":
         from oracle import load
+
+        from cs_dmri_amd import fista_batch
         orc = load()
         A = orc.design_matrix(wl.radial_order, wl.zeta, g.bvals, g.bvecs, g.big_delta, g.small_delta)
         S = wl.data[wl.mask]
+        fit_kw = dict(alpha_mode="fixed", alpha=1e-3, max_iter=args.max_iter, tol=1e-6)
+        threads, avail, host = cpu_threads()
         pick = np.linspace(0, V - 1, args.cpu_sample).astype(np.int64)
-        sub = S[pick].reshape(len(pick), 1, 1, N)
-        threads = min(os.cpu_count() or 1, 16)
-        t0 = time.perf_counter()
-        ref = orc.fit_volume(sub, None, A, alpha_mode="fixed", alpha=1e-3, max_iter=args.max_iter, tol=1e-6,
-                             n_threads=threads)
-        cpu_wall = time.perf_counter() - t0
-        cr = ref["coefficients"].reshape(len(pick), K).astype(np.float64)
-        cg = coef[pick].astype(np.float64)
-        err = np.linalg.norm(cg - cr, axis=1) / np.maximum(np.linalg.norm(cr, axis=1), 1e-30)
-        result["cpu_baseline"] = {"value": round(len(pick) / cpu_wall, 2), "unit": "voxel-fits/s",
-                                  "cores": threads, "kind": "port",
-                                  "sample": f"{len(pick)} voxels evenly spaced over the mask, same fit settings, "
-                                            f"FP64 oracle (C++), {cpu_wall:.1f} s wall"}
-        # trajectory parity: the same voxels at an equal iteration count (tol = 0)
-        from cs_dmri_amd import fista_batch
-        tp = pick[:: max(1, len(pick) // 256)][:256]
+        rate, cpu_wall, ref = cpu_fit_rate(orc, S[pick], A, N, K, threads=threads, **fit_kw)
+        one = pick[:: max(1, len(pick) // args.cpu_single_sample)][: args.cpu_single_sample]
+        rate1, wall1, _ = cpu_fit_rate(orc, S[one], A, N, K, threads=1, **fit_kw)
+        result["cpu_baseline"] = {
+            "value": round(rate, 2), "unit": "voxel-fits/s", "cores": threads, "threads": threads,
+            "host_cores": host, "cpus_available": avail, "single_thread_value": round(rate1, 2), "kind": "port",
+            "sample": f"{len(pick)} voxels evenly spaced over the mask, same fit settings, FP64 oracle (C++), one "
+                      f"voxel per task on {threads} threads, {cpu_wall:.1f} s wall; single thread: {len(one)} of "
+                      f"those voxels, {wall1:.1f} s"}
+        cr = ref["coefficients"].reshape(len(pick), K)
+        itr = ref["iterations"].ravel()
+        acc = {"stopping_rule": err_stats(coef[pick], cr, its[pick], itr)}
+        # the FP32 form on the same voxels: the yardstick for the mixed forms
+        if args.precision != "fp32":
+            c32, d32 = fista_batch(A, S[pick], 1e-3, max_iter=args.max_iter, tol=1e-6, precision="fp32")
+            acc["stopping_rule_fp32_form"] = err_stats(c32, cr, d32["iterations"], itr)
+        # trajectory parity: equal iteration counts (tol = 0)
+        tp = pick[:: max(1, len(pick) // args.equal_iter_sample)][: args.equal_iter_sample]
         ct, _ = fista_batch(A, S[tp], 1e-3, max_iter=args.max_iter, tol=0.0, precision=args.precision,
                             refresh_every=args.refresh)
-        rt = orc.fit_volume(S[tp].reshape(len(tp), 1, 1, N), None, A, alpha_mode="fixed", alpha=1e-3,
-                            max_iter=args.max_iter, tol=0.0, n_threads=threads)
-        rtc = rt["coefficients"].reshape(len(tp), K).astype(np.float64)
-        et = np.linalg.norm(ct.astype(np.float64) - rtc, axis=1) / np.maximum(np.linalg.norm(rtc, axis=1), 1e-30)
-        result["accuracy_vs_fp64"] = {
-            "equal_iterations": {"sample": len(tp), "iterations": args.max_iter,
-                                 "coef_rel_err_max": float(et.max()), "coef_rel_err_median": float(np.median(et))},
-            "stopping_rule": {"sample": len(pick), "coef_rel_err_median": float(np.median(err)),
-                              "coef_rel_err_p99": float(np.quantile(err, 0.99)),
-                              "coef_rel_err_max": float(err.max()),
-                              "frac_le_1e-4": float(np.mean(err <= 1e-4)),
-                              "iterations_equal_frac": float(np.mean(ref["iterations"].ravel() == its[pick]))}}
+        _, _, rt = cpu_fit_rate(orc, S[tp], A, N, K, threads=threads, alpha_mode="fixed", alpha=1e-3,
+                                max_iter=args.max_iter, tol=0.0)
+        eq = err_stats(ct, rt["coefficients"].reshape(len(tp), K))
+        eq["iterations"] = args.max_iter
+        acc["equal_iterations"] = eq
+        result["accuracy_vs_fp64"] = acc
     if postfit is not None:
         result["postfit"] = postfit
     print(json.dumps(result), flush=True)

@@ -123,13 +123,18 @@ void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hi
 void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, bool transpose, double scale,
                           int rows_p, int cols_p, float* d_frag, hipStream_t s);
 // -c G in FP16 / BF16 16x16x32 fragment order (mixed kernel increments);
-// part 1 packs the FP16 rounding remainder scaled by 2^12 (split refresh).
+// part 1 packs the FP16 rounding remainder scaled by 2^12 (split refresh),
+// part 2 the remainder unscaled in the same type (split increments).
 void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s,
                         int part = 0);
 // dst (rows x cols_p, FP64) = src (rows x cols) zero-padded on the right.
 void launch_pad_cols(const double* d_src, int rows, int cols, int cols_p, double* d_dst, hipStream_t s);
 
 // ---------------------------------------------------------------- fits ----
+// Largest padded coefficient count whose mixed-precision increments apply
+// both halves of the split operator (hi and lo, 2 x Kp^2 x 2 bytes of LDS).
+constexpr int kMixedSplitMaxKp = 160;
+
 struct FistaLaunch {
     // signals: row v of the fit is data[vox[v] * sig_stride ...] (N floats);
     // vox == nullptr means row v itself.
@@ -144,6 +149,7 @@ struct FistaLaunch {
     const float* Afrag;
     const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
     const uint16_t* Mlow2;  // FP16 remainder of -cG x 2^12 (split refresh, FP16 only)
+    const uint16_t* Mlow3;  // remainder -cG - lowp(-cG), unscaled (split increments, padded K <= 160)
     const double* A64p;     // FP64 design zero-padded to N x Kp (right-hand sides)
     const double* c_dev;    // step / N on the device (read by the right-hand-side kernel)
     float* Bbuf;            // scratch [n_vox][Kp]: b = c A^T s

@@ -878,6 +878,7 @@ struct MixedArgs {
     const floatx4* M32;     // -cG, 16x16x4 f32 fragments (MB*MB*64)
     const uint4* M16;       // -cG, 16x16x32 f16/bf16 fragments (MB*KC*64)
     const uint4* M16lo;     // FP16 remainder of -cG, x 2^12 (split refresh)
+    const uint4* Mlo_inc;   // remainder -cG - lowp(-cG), unscaled (split increments)
     const float* mom;       // momentum (t_k - 1)/t_{k+1}, k < max_iter
     float thr, step, ratio, alpha_fixed;
     uint32_t max_iter;
@@ -930,20 +931,36 @@ __device__ __forceinline__ float pow2_to(float bound, int target) {
     return ldexpf(1.f, target - e);
 }
 
-// B_LDS: keep each slot's b in LDS (needed by the refresh); otherwise in
-// registers (order 10, where the FP16 matrix alone takes 128 KB of LDS).
-// SPLIT: refresh with FP16 hi/lo splits (three FP16 MFMA passes) instead of
-// the FP32 MFMA (FP16 increments only).
-template <int MB, int WAVES, bool BF16, bool B_LDS, bool NONNEG, bool SPLIT, bool PATH>
+// SPLITINC: the increments apply both halves of the split operator,
+// lowp(-cG) and the unscaled remainder -cG - lowp(-cG) (FP16 subnormal where
+// |cG| < 2^-2; the FP16 MFMA takes subnormal inputs as they are), chained on
+// one accumulator, so the operator acting on dy carries ~2^-22 (FP16) /
+// 2^-17 (BF16) relative error instead of the single rounding's 2^-12 / 2^-9. That error would otherwise shift the
+// small eigenvalues of c G, the slow modes that decide the iteration at which
+// |dx| / |x| crosses tol, and so move the stopping iteration away from the
+// FP64 fit's. Both halves live in LDS (2 x 51 KB at order 8); each slot's b
+// is read from global memory when the refresh needs it. Without SPLITINC
+// (order 10, whose 128 KB FP16 matrix fills LDS alone) only lowp(-cG) is
+// applied and b stays in registers.
+// Each increment's MFMA chain starts from zero and is added to acc with one
+// rounding, so the chain's roundings scale with the increment, not with acc.
+// SPLIT: refresh with FP16 hi/lo splits (three FP16 MFMA passes, ~2^-22)
+// instead of the FP32 MFMA (FP16 increments only). Measured on the whole-brain
+// fit, the FP32 refresh moves fewer stopping iterations off the FP64 fit's
+// (92.8% equal against 89.5% at refresh 40) for 6% more solve time, so the
+// split refresh is kept only where the increments are not split.
+template <int MB, int WAVES, bool BF16, bool SPLITINC, bool NONNEG, bool SPLIT, bool PATH>
 __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs a) {
     constexpr int KC = MB / 2;
-    constexpr int BSTRIDE = 16 * MB + 4;  // padded slot stride (floats) against LDS bank conflicts
+    constexpr int NF = MB * KC * 64;  // uint4 fragments of one operator half
     extern __shared__ uint4 smem[];
-    uint4* Ml = smem;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint4* Ml = smem;  // lowp(-cG), then (SPLITINC) the remainder
+    const int lane = threadIdx.x & 63;
     const int h = lane >> 4, j = lane & 15;
-    float* bsl = reinterpret_cast<float*>(smem + MB * KC * 64) + (wave * 16 + j) * BSTRIDE + 4 * h;
-    for (int i = threadIdx.x; i < MB * KC * 64; i += 64 * WAVES) Ml[i] = a.M16[i];
+    for (int i = threadIdx.x; i < NF; i += 64 * WAVES) {
+        Ml[i] = a.M16[i];
+        if constexpr (SPLITINC) Ml[NF + i] = a.Mlo_inc[i];
+    }
     __syncthreads();
 
     floatx4 x[MB], y[MB], acc[MB], breg[MB];
@@ -978,12 +995,8 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
         for (int mb = 0; mb < MB; ++mb) {
             floatx4 bq = floatx4{0.f, 0.f, 0.f, 0.f};
             if (need && alive) bq = *reinterpret_cast<const floatx4*>(a.B + vid * a.Kp + 16 * mb + 4 * h);
-            if (need) {
-                if constexpr (B_LDS)
-                    *reinterpret_cast<floatx4*>(bsl + 16 * mb) = bq;
-                else
-                    breg[mb] = bq;
-            }
+            if constexpr (!SPLITINC)
+                if (need) breg[mb] = bq;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 m = fmaxf(m, fabsf(bq[r]));
@@ -1182,6 +1195,14 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                 // a 32-bit per-lane offset (one VALU add per load).
                 const uint32_t loff = (uint32_t(moff) + uint32_t(lane)) * 16u;
                 const char* lo16 = reinterpret_cast<const char*>(a.M16lo);
+                if constexpr (SPLITINC) {
+                    // b of the slot's voxel into acc (dead until rewritten
+                    // below); the loads complete behind the VALU work
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb)
+                        acc[mb] = alive ? *reinterpret_cast<const floatx4*>(a.B + vid * a.Kp + 16 * mb + 4 * h)
+                                        : floatx4{0.f, 0.f, 0.f, 0.f};
+                }
                 constexpr int LA = MB < 2 ? MB : 2;  // blocks of remainder fragments in flight
                 uint4 glo[LA + 1][KC];
                 if constexpr (SPLIT) {
@@ -1250,11 +1271,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                             c1 = mfma16<false>(mh, yl[kc], c1);
                             c2 = mfma16<false>(glo[mb % (LA + 1)][kc], yh[kc], c2);
                         }
-                        floatx4 bq;
-                        if constexpr (B_LDS)
-                            bq = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
-                        else
-                            bq = breg[mb];
+                        const floatx4 bq = SPLITINC ? acc[mb] : breg[mb];
                         acc[mb] = bq * sc + (c1 + c2 * (1.f / 4096.f)) * isy;
                         // one output block at a time: bounds the registers in flight
                         __builtin_amdgcn_sched_barrier(0);
@@ -1262,11 +1279,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                 } else {
 #pragma unroll
                     for (int mb = 0; mb < MB; ++mb) {
-                        floatx4 c;
-                        if constexpr (B_LDS)
-                            c = *reinterpret_cast<const floatx4*>(bsl + 16 * mb);
-                        else
-                            c = breg[mb];
+                        floatx4 c = SPLITINC ? acc[mb] : breg[mb];
                         c *= sc;
 #pragma unroll
                         for (int kb = 0; kb < MB; ++kb) {
@@ -1277,6 +1290,35 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                         }
                         acc[mb] = c;
                     }
+                }
+            } else if constexpr (SPLITINC) {
+                // acc += (lowp(-cG) + remainder) lowp(dy), both halves on one
+                // chain. The schedule keeps the next output block's 2 KC
+                // fragments in flight from LDS while one block's MFMAs run.
+                uint4 fr[2 * MB * KC];
+#pragma unroll
+                for (int i = 0; i < MB * KC; ++i) {
+                    fr[2 * i] = Ml[moff + i * 64 + lane];
+                    fr[2 * i + 1] = Ml[moff + NF + i * 64 + lane];
+                }
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    // the chain starts from zero, so its roundings are relative
+                    // to the increment, not to acc; one rounding adds it
+                    floatx4 c = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kc = 0; kc < KC; ++kc) {
+                        const uint4 dv = uint4{dyh[kc][0], dyh[kc][1], dyh[kc][2], dyh[kc][3]};
+                        c = mfma16<BF16>(fr[2 * (mb * KC + kc)], dv, c);
+                        c = mfma16<BF16>(fr[2 * (mb * KC + kc) + 1], dv, c);
+                    }
+                    acc[mb] += c;
+                }
+                __builtin_amdgcn_sched_group_barrier(0x100, 2 * KC, 0);  // DS reads
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 2 * KC, 0);  // MFMA
+                    if (mb + 1 < MB) __builtin_amdgcn_sched_group_barrier(0x100, 2 * KC, 0);
                 }
             } else {
                 // acc += (-cG) fp16(dy). The schedule keeps the fragments of
@@ -1477,8 +1519,8 @@ void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t
 
 template <int MB, bool BF16>
 void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
-    constexpr bool B_LDS = MB <= 12;
-    constexpr int WAVES = B_LDS ? 8 : 4;
+    constexpr bool SPLITINC = 16 * MB <= kMixedSplitMaxKp;
+    constexpr int WAVES = SPLITINC ? 8 : 4;
     constexpr int KC = MB / 2;
     MixedArgs m{};
     m.vox = L.vox;
@@ -1489,6 +1531,8 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.M32 = reinterpret_cast<const floatx4*>(L.Mfrag);
     m.M16 = reinterpret_cast<const uint4*>(L.Mlow);
     m.M16lo = reinterpret_cast<const uint4*>(L.Mlow2);
+    m.Mlo_inc = reinterpret_cast<const uint4*>(L.Mlow3);
+    if (SPLITINC && !m.Mlo_inc) throw Error{SHOREFIT_E_DEVICE, "mixed-precision FISTA: split operator not packed"};
     m.mom = L.mom;
     m.thr = L.thr;
     m.step = L.step;
@@ -1508,12 +1552,14 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.itspath = L.itspath;
     m.convpath = L.convpath;
     const bool path = L.path_n > 0;
-    const size_t lds = size_t(MB) * KC * 64 * 16 + (B_LDS ? size_t(WAVES) * 16 * (16 * MB + 4) * 4 : 0);
-    constexpr bool SPLIT = !BF16;
-    auto kern = path ? (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true, SPLIT, true>
-                                 : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false, SPLIT, true>)
-                     : (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, B_LDS, true, SPLIT, false>
-                                 : fista_mixed_kernel<MB, WAVES, BF16, B_LDS, false, SPLIT, false>);
+    const size_t lds = size_t(MB) * KC * 64 * 16 * (SPLITINC ? 2 : 1);
+    // refresh on the FP32 MFMA, except FP16 without split increments (order
+    // 10), which refreshes from FP16 hi/lo splits
+    constexpr bool SPLIT = !BF16 && !SPLITINC;
+    auto kern = path ? (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, true, SPLIT, true>
+                                 : fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, false, SPLIT, true>)
+                     : (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, true, SPLIT, false>
+                                 : fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, false, SPLIT, false>);
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(lds));
     int dev = 0, cus = 0, per_cu = 0;

@@ -398,7 +398,9 @@ __global__ void pack_M_kernel(const double* __restrict__ G, int K, int Kp, doubl
 // p(v) = 32kc + 16(v/4) + 4h + v%4 is the row of Y that lane (h, j) of the
 // accumulator layout holds in register v%4 of block 2kc + v/4.
 // part 0: the FP16 (BF16) rounding of -cG; part 1 (FP16 only): the rounding
-// remainder -cG - fp16(-cG), scaled by 2^12 so it stays in FP16's normal range.
+// remainder -cG - fp16(-cG), scaled by 2^12 so it stays in FP16's normal range
+// (the exact refresh); part 2: the remainder -cG - lowp(-cG) unscaled, in FP16
+// (subnormal where |cG| < 2^-2) or BF16 (the increments).
 __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, double c, int bf16, int part,
                                    uint16_t* __restrict__ frag) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -413,10 +415,12 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
     if (i < K && p < K) x = -c * G[size_t(i) * K + p];
     if (bf16) {
         __hip_bfloat16 b = __float2bfloat16(float(x));
+        if (part == 2) b = __float2bfloat16(float(x - double(__bfloat162float(b))));
         frag[t] = *reinterpret_cast<uint16_t*>(&b);
     } else {
         __half hv = __float2half(float(x));
         if (part == 1) hv = __float2half(float((x - double(__half2float(hv))) * 4096.0));
+        if (part == 2) hv = __float2half(float(x - double(__half2float(hv))));
         frag[t] = *reinterpret_cast<uint16_t*>(&hv);
     }
 }

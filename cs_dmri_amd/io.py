@@ -187,12 +187,12 @@ def load_mask(path, expected_shape) -> np.ndarray:
     return np.ascontiguousarray(v > 0.0)
 
 
-def default_mask_from_b0(data, bvals) -> np.ndarray:
-    """Voxels whose mean b <= 50 signal (all volumes if there is no b0)
-    exceeds 1% of the largest such mean; the mean is an f32 sum in frame
+def default_mask_from_b0(data, bvals, b0_threshold: float = B0_THRESHOLD) -> np.ndarray:
+    """Voxels whose mean b <= b0_threshold signal (all volumes if there is no
+    b0) exceeds 1% of the largest such mean; the mean is an f32 sum in frame
     order over the frame count (src/io/dwi.rs:347-365)."""
     d = data if isinstance(data, np.ndarray) else data.cpu().numpy()
-    b0 = [t for t, b in enumerate(bvals) if b <= B0_THRESHOLD] or list(range(d.shape[3]))
+    b0 = [t for t, b in enumerate(bvals) if b <= b0_threshold] or list(range(d.shape[3]))
     m = np.zeros(d.shape[:3], np.float32)
     for t in b0:
         m += d[..., t]
@@ -352,16 +352,25 @@ def atomic_write(final, overwrite: bool, write) -> None:
         raise
 
 
+def _grid_header(reference) -> np.ndarray:
+    """The header whose grid a new file takes: a NIfTI path, or a header
+    array (``nifti.read_header`` / ``nifti.new_header``)."""
+    if isinstance(reference, (np.ndarray, np.void)) and reference.dtype.names:
+        return reference
+    return nf.read_header(reference)
+
+
 def write_coefficients(path, coeffs, reference_nifti, sidecar: dict, overwrite: bool = False) -> None:
     """Coefficient NIfTI (X, Y, Z, K) float32 with the reference DWI's
-    spatial header, and its JSON sidecar, as an atomic pair
-    (src/io/coeffs.rs:112-160). ``coeffs`` may be a device tensor."""
+    spatial header (a path or a header array), and its JSON sidecar, as an
+    atomic pair (src/io/coeffs.rs:112-160). ``coeffs`` may be a device
+    tensor."""
     shp = tuple(coeffs.shape)
     if len(shp) != 4:
         raise ValueError(f"coefficients must be 4D (X, Y, Z, K), got {shp}")
     if "n_coefficients" in sidecar and int(sidecar["n_coefficients"]) != shp[3]:
         raise ValueError(f"sidecar n_coefficients ({sidecar['n_coefficients']}) != coefficient count ({shp[3]})")
-    header = nf.derived_header(nf.read_header(reference_nifti), shp, 16, 32)
+    header = nf.derived_header(_grid_header(reference_nifti), shp, 16, 32)
     data = to_file_order(coeffs)
     text = json.dumps(sidecar, indent=2)
     atomic_write_pair(path, sidecar_path_for(path), overwrite,
@@ -381,12 +390,13 @@ def read_coefficients(path, device=None):
 
 
 def write_map(path, values, reference_nifti, overwrite: bool = False) -> None:
-    """3D float32 map on the reference's grid (src/io/output.rs:125-141)."""
+    """3D float32 map on the reference's grid (a path or a header array;
+    src/io/output.rs:125-141)."""
     v = values if isinstance(values, np.ndarray) else values.cpu().numpy()
     v = np.asarray(v, np.float32)
     if v.ndim != 3:
         raise ValueError(f"map must be 3D, got {v.shape}")
-    header = nf.derived_header(nf.read_header(reference_nifti), v.shape, 16, 32)
+    header = nf.derived_header(_grid_header(reference_nifti), v.shape, 16, 32)
     atomic_write(path, overwrite, lambda tmp: nf.save(tmp, header, np.ascontiguousarray(v.transpose(2, 1, 0))))
 
 

@@ -32,7 +32,7 @@ def design_matrix(bvals, bvecs, *, radial_order=6, zeta=700.0, big_delta=None, s
 
 
 def fista_batch(A, signals, alpha, *, max_iter=1000, tol=1e-6, non_negative=False, precision="fp32",
-                refresh_every=10):
+                refresh_every=None):
     """FistaSolver::fit (src/solver/fista.rs:38-150) for every row of ``signals`` against design ``A``.
 
     Returns ``(coef (V,K) f32, dict(iterations, converged, residual_l2, lipschitz))``.
@@ -50,7 +50,7 @@ def fista_batch(A, signals, alpha, *, max_iter=1000, tol=1e-6, non_negative=Fals
     lip = C.c_double()
     L.check(L.lib().shorefit_fista_batch(
         L.ptr(A, L._pd), N, K, L.ptr(S, L._pf), V, float(alpha), int(max_iter), float(tol), int(non_negative),
-        L.PRECISION[precision], int(refresh_every), L.ptr(coef, L._pf), its.ctypes.data_as(C.POINTER(C.c_uint32)),
+        L.PRECISION[precision], 0 if refresh_every is None else int(refresh_every), L.ptr(coef, L._pf), its.ctypes.data_as(C.POINTER(C.c_uint32)),
         conv.ctypes.data_as(C.POINTER(C.c_uint8)), L.ptr(res, L._pf), C.byref(lip)))
     return coef, {"iterations": its, "converged": conv.astype(bool), "residual_l2": res, "lipschitz": lip.value}
 
@@ -99,7 +99,7 @@ def _spec(radial_order, zeta, regularization, alpha_mode, alpha, alpha_ratio, pa
     s.compute_diagnostics = int(bool(diagnostics))
     s.allow_single_shell = int(bool(allow_single_shell))
     s.precision = L.PRECISION[precision]
-    s.refresh_every = int(refresh_every)
+    s.refresh_every = 0 if refresh_every is None else int(refresh_every)
     return s
 
 
@@ -148,7 +148,7 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
               alpha=1.0, alpha_ratio=1e-3, path_n_alphas=20, path_eps=None, slack=0.05, max_iter=1000, tol=1e-6,
               non_negative=False, lambda_n=1e-8, lambda_l=1e-8, nonneg_max_iter=200, nonneg_tol=1e-9,
               nonneg_epsilon=1e-10, diagnostics=False, allow_single_shell=False, n_threads=None,
-              precision="fp32", refresh_every=10, graddev=None, bvecs_vox=None, frame_rotation=None,
+              precision="fp32", refresh_every=None, graddev=None, bvecs_vox=None, frame_rotation=None,
               graddev_variant=-1) -> dict:
     """Drop-in for ``_cs_dmri.shore_fit`` (python/src/models.rs:70-198).
 

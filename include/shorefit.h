@@ -76,7 +76,8 @@ typedef struct ShorefitSpec {
     int32_t allow_single_shell;
     /* device knobs */
     int32_t precision;       /* SHOREFIT_PREC_* */
-    int32_t refresh_every;   /* FP32 refresh period of the mixed-precision kernel (>= 1) */
+    int32_t refresh_every;   /* exact-refresh period of the mixed-precision kernel; <= 0 = the default
+                                for the arithmetic (FP16: 40 at order <= 8, else 10; BF16: 10) */
     int32_t reserved[6];
 } ShorefitSpec;
 
@@ -130,7 +131,8 @@ typedef struct ShorefitOutput {
 
 /* Fill *spec with the cs-fit / ShoreModel defaults (src/fit.rs:647-669):
  * order 6, zeta 700, L1 + L2-anchored path (20 alphas, eps 1e-4, slack 0.05),
- * max_iter 1000, tol 1e-6, lambda_n = lambda_l = 1e-8, FP32. */
+ * max_iter 1000, tol 1e-6, lambda_n = lambda_l = 1e-8, FP32, refresh_every 0
+ * (the arithmetic's default). */
 void shorefit_spec_default(ShorefitSpec* spec);
 
 /* Whole-volume fit: replaces fit_shore (src/fit.rs:687-805). stream is a

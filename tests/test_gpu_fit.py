@@ -199,12 +199,44 @@ def test_empty_mask_and_determinism():
     np.testing.assert_array_equal(s1["coefficients"][3, 2, 1], a["coefficients"][3, 2, 1])
 
 
-def test_shore_model_front_end():
+def test_shore_model_front_end(tmp_path):
+    # ShoreModel.fit is shore_fit on the same arrays (python/cs_dmri/shore.py:80-95)
     gt, data, mask = _small_volume()
     gtab = sf.GradientTable(gt.bvals, gt.bvecs, big_delta=gt.big_delta, small_delta=gt.small_delta)
-    fit = sf.ShoreModel(gtab, radial_order=4, alpha_mode="fixed", alpha=1e-3, max_iter=200).fit(data, mask)
-    assert fit.coefficients.shape == mask.shape + (29,) and fit.frame == "image"
-    assert fit.r2 is not None and fit.iterations is not None and fit.radial_order == 4
+    kw = dict(radial_order=4, alpha_mode="fixed", alpha=1e-3, max_iter=200)
+    fit = sf.ShoreModel(gtab, **kw).fit(data, mask)
+    ref = sf.shore_fit(data, gt.bvals, gt.bvecs, mask, big_delta=gt.big_delta, small_delta=gt.small_delta,
+                       diagnostics=True, **kw)
+    np.testing.assert_array_equal(fit.coefficients, ref["coefficients"])
+    for k in ("r2", "rmse", "residual_l2", "iterations", "regularization_kind", "alpha", "bic"):
+        np.testing.assert_array_equal(getattr(fit, k), ref[k], err_msg=k)
+    assert fit.rss_l2 is None and fit.frame == "image" and fit.radial_order == 4
+    assert fit.sidecar == ref["sidecar"] and fit.sidecar["bvec_frame"] == "image-axis"
+    np.testing.assert_allclose(fit.sparsity[mask], (ref["coefficients"][mask] != 0).mean(-1), rtol=1e-6)
+    assert "ShoreFit" in repr(fit) and f"voxels={int(mask.sum())}" in repr(fit)
+    # world frame: the bvecs go through the affine's rotation
     aff = np.diag([-2.0, 2.0, 2.0, 1.0])
-    fw = sf.ShoreModel(gtab, radial_order=4, alpha_mode="fixed", alpha=1e-3, max_iter=200).fit(data, mask, affine=aff)
+    fw = sf.ShoreModel(gtab, **kw).fit(data, mask, affine=aff)
+    rw = sf.shore_fit(data, gt.bvals, gt.bvecs * np.array([-1.0, 1.0, 1.0]), mask, big_delta=gt.big_delta,
+                      small_delta=gt.small_delta, bvec_frame="world-ras", diagnostics=True, **kw)
     assert fw.frame == "world" and fw.sidecar["bvec_frame"] == "world-ras"
+    np.testing.assert_array_equal(fw.coefficients, rw["coefficients"])
+    # without a mask an array gets the b0 fallback mask (src/io/dwi.rs:347-365)
+    from cs_dmri_amd.io import default_mask_from_b0
+    fb = sf.ShoreModel(gtab, **kw).fit(data)
+    np.testing.assert_array_equal(fb.mask, default_mask_from_b0(data, gt.bvals))
+    # save / load round trip in the cs-fit layout; existing files are refused first
+    p = tmp_path / "fit.nii.gz"
+    fw.save(p)
+    for suffix in ("_r2", "_rmse", "_alpha", "_bic", "_sparsity"):
+        assert (tmp_path / f"fit{suffix}.nii.gz").exists(), suffix
+    assert (tmp_path / "fit.json").exists() and not (tmp_path / "fit_rss_l2.nii.gz").exists()
+    with pytest.raises(FileExistsError):
+        fw.save(p)
+    back = sf.ShoreFit.load(p, mask=mask)
+    np.testing.assert_array_equal(back.coefficients, fw.coefficients)
+    np.testing.assert_array_equal(back.r2, fw.r2)
+    np.testing.assert_array_equal(back.bic, fw.bic)
+    assert back.frame == "world" and back.sidecar == fw.sidecar and back.iterations is None
+    np.testing.assert_allclose(back.affine, aff)
+    np.testing.assert_array_equal(back.odf_sh(), fw.odf_sh())

@@ -1,14 +1,20 @@
 """GPU parity of the mixed-precision persistent FISTA kernel.
 
-FP16 (or BF16) MFMA increments of acc = b - cG y with an exact FP32 refresh
-every `refresh_every` iterations, per-voxel power-of-two scaling, and
-persistent slots refilled from a work counter.
+FP16 (or BF16) MFMA increments of acc = b - cG y through both halves of the
+split operator, an exact refresh every `refresh_every` iterations, per-voxel
+power-of-two scaling, and persistent slots refilled from a work counter.
 
-Tolerances (coefficients relative to the FP64 oracle, per voxel):
-* FP16, refresh 10, equal iteration counts (tol = 0), 500 iterations:
-  max <= 1e-4 (the north-star bound for the mixed-precision fit).
-* BF16, refresh 10, same: max <= 1e-3 (8-bit operand mantissa).
-* With the stopping rule: median <= 1e-4, p99 <= 1e-3.
+Tolerances (coefficients relative to the FP64 oracle, per voxel) follow the
+headline metric, coefficient error <= 1e-4 against FP64:
+* equal iteration counts (tol = 0, 500 iterations), FP16 and BF16 at refresh
+  5 / 10 / 20 / 40, orders 6 and 8: max <= 1e-4.
+* the stopping rule (tol = 1e-6) on whole-brain-table signals at order 8,
+  against the FP32 form on the same voxels (the yardstick; it also stops off
+  the FP64 iteration for some voxels): p99 <= 1e-4, the fraction within 1e-4
+  no more than 1 point below the FP32 form's, and the FP64 stopping iteration
+  reproduced for >= 88% of voxels. Measured: FP16 is 0.3 (32,768 voxels) to
+  0.6 (these 4,096) points below FP32 within 1e-4, and stops at the FP64
+  iteration for 90-93% of voxels against FP32's 95%.
 """
 
 import numpy as np
@@ -35,14 +41,39 @@ def cfg1(orc):
     return wl, A, S, ref500
 
 
-@pytest.mark.parametrize("precision,refresh,bound", [("fp16", 10, 1e-4), ("fp16", 5, 1e-4), ("bf16", 10, 1e-3)])
-def test_mixed_equal_iterations(cfg1, precision, refresh, bound):
+@pytest.mark.parametrize("precision,refresh", [("fp16", 5), ("fp16", 10), ("fp16", 20), ("fp16", 40),
+                                               ("bf16", 5), ("bf16", 10), ("bf16", 20)])
+def test_mixed_equal_iterations(cfg1, precision, refresh):
     wl, A, S, ref = cfg1
     coef, d = sf.fista_batch(A, S[:512], 1e-3, max_iter=500, tol=0.0, precision=precision, refresh_every=refresh)
     assert (d["iterations"] == 500).all() and not d["converged"].any()
     err = _rel(coef.astype(np.float64), ref)
-    print(f"{precision} k={refresh}: max {err.max():.2e} median {np.median(err):.2e}")
-    assert err.max() <= bound
+    print(f"{precision} k={refresh}: max {err.max():.2e} p99 {np.quantile(err, 0.99):.2e} median {np.median(err):.2e}")
+    assert err.max() <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def cfg2_sample(orc):
+    gt, S = W.config2_signals(4096, seed=11)
+    A = orc.design_matrix(8, 700.0, gt.bvals, gt.bvecs, gt.big_delta, gt.small_delta)
+    ref = orc.fit_volume(S.reshape(len(S), 1, 1, -1), None, A, alpha_mode="fixed", alpha=1e-3, max_iter=500,
+                         tol=1e-6, n_threads=8)
+    return A, S, ref["coefficients"].reshape(len(S), -1).astype(np.float64), ref["iterations"].ravel()
+
+
+@pytest.mark.parametrize("precision,refresh", [("fp16", None), ("fp16", 10), ("bf16", None)])
+def test_mixed_stopping_rule_against_fp32_form(cfg2_sample, precision, refresh):
+    A, S, ref, itr = cfg2_sample
+    c32, d32 = sf.fista_batch(A, S, 1e-3, max_iter=500, tol=1e-6, precision="fp32")
+    cm, dm = sf.fista_batch(A, S, 1e-3, max_iter=500, tol=1e-6, precision=precision, refresh_every=refresh)
+    e32, em = _rel(c32.astype(np.float64), ref), _rel(cm.astype(np.float64), ref)
+    f32, fm = np.mean(e32 <= 1e-4), np.mean(em <= 1e-4)
+    eq32, eqm = np.mean(d32["iterations"] == itr), np.mean(dm["iterations"] == itr)
+    print(f"{precision} k={refresh}: within 1e-4 {fm:.4f} (fp32 {f32:.4f}), p99 {np.quantile(em, 0.99):.2e} "
+          f"(fp32 {np.quantile(e32, 0.99):.2e}), FP64 iteration {eqm:.3f} (fp32 {eq32:.3f})")
+    assert np.quantile(em, 0.99) <= 1e-4
+    assert fm >= f32 - 0.01
+    assert eqm >= 0.88
 
 
 def test_mixed_stopping_rule_volume(orc, cfg1):

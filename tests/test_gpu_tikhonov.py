@@ -15,7 +15,7 @@ Tolerances
 * normal equations (tests/optimization_kkt.rs:164-218 restated on the
   device's FP32 coefficients, evaluated in FP64): the relative residual
   |(A^T A + Lambda) c - A^T s| / |A^T s| is below 1e-5, the relative
-  scale of FP32 coefficient parity (measured: 2.5e-7, against 1.3e-8 for the
+  scale of FP32 coefficient parity (measured: at most 6.4e-7, against 5e-8 for the
   oracle's FP64 solution merely rounded to float32; the apply c = P s
   accumulates 258 FP32 products per coefficient). The oracle's own FP64
   solution meets the reference's 1e-10 on this design
