@@ -28,6 +28,7 @@ EXPORTS = (
     "shorefit_fit_graddev", "shorefit_graddev_variants", "shorefit_graddev_tune", "shorefit_graddev_gram",
     "shorefit_design_create", "shorefit_design_destroy", "shorefit_design_info", "shorefit_lasso_forward",
     "shorefit_lasso_adjoint", "shorefit_volume_to_voxel_major", "shorefit_volume_to_file_order",
+    "shorefit_lasso_forward_noalloc",
 )
 
 
@@ -162,6 +163,8 @@ def _load() -> C.CDLL:
     lib.shorefit_lasso_forward.argtypes = [_vp, _vp, C.c_int64, _vp, C.c_double, _vp, C.c_uint32, C.c_double,
                                            C.c_int32, _vp, _vp, _vp]
     lib.shorefit_lasso_forward.restype = C.c_int
+    lib.shorefit_lasso_forward_noalloc.argtypes = lib.shorefit_lasso_forward.argtypes
+    lib.shorefit_lasso_forward_noalloc.restype = C.c_int
     lib.shorefit_lasso_adjoint.argtypes = [_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp]
     lib.shorefit_lasso_adjoint.restype = C.c_int
     lib.shorefit_volume_to_voxel_major.argtypes = [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_int64,

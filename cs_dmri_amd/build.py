@@ -18,7 +18,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libshorefit.so"
-SOURCES = ["setup.hip", "fista.hip", "postfit.hip", "graddev.hip", "adjoint.hip", "volume.hip", "api.cpp"]
+SOURCES = ["setup.hip", "fista.hip", "postfit.hip", "graddev.hip", "adjoint.hip", "volume.hip", "noalloc.hip",
+           "api.cpp"]
 ARCH = "gfx950"
 
 

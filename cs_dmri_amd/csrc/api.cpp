@@ -1160,51 +1160,108 @@ int shorefit_design_info(const ShorefitDesign* h, int32_t* n_rows, int32_t* n_co
     });
 }
 
+}  // extern "C"
+
+namespace {
+// The FISTA launch of a lasso forward on a kept design (both entry points);
+// the caller supplies the right-hand-side, momentum and counter buffers.
+FistaLaunch lasso_launch(const ShorefitDesign* h, const float* signals, int64_t V, const float* alpha,
+                         double alpha_scalar, const float* warm, uint32_t max_iter, double tol, int32_t non_negative,
+                         float* coef, uint32_t* iterations) {
+    const Design& d = h->d;
+    FistaLaunch L{};
+    L.data = signals;
+    L.vox = nullptr;
+    L.n_vox = V;
+    L.N = d.N;
+    L.sig_stride = d.N;
+    L.K = d.K;
+    L.Kp = d.Kp;
+    L.Np = d.Np;
+    L.alpha = alpha_scalar;
+    L.max_iter = max_iter;
+    L.tol = float(tol);
+    L.nonneg = non_negative;
+    L.precision = SHOREFIT_PREC_FP32;
+    L.coef = coef;
+    L.iters = iterations;
+    L.A64p = h->ops.Ap.p;
+    L.c_dev = h->work.sc.p + 1;
+    L.Mfrag = h->ops.M.p;
+    L.Afrag = h->ops.A.p;
+    L.step = float(h->ops.step);
+    L.step64 = h->ops.step;
+    L.thr = float(h->ops.step * alpha_scalar);
+    L.alpha_vox = alpha;
+    L.warm = warm;
+    // persistent FP32 form: voxels of uneven convergence refill their slots
+    L.persist = true;
+    return L;
+}
+
+void check_lasso_args(const ShorefitDesign* h, const float* signals, int64_t n_signals, float* coef,
+                      const char* who) {
+    if (!h || (n_signals > 0 && (!signals || !coef)) || n_signals < 0)
+        throw Error{SHOREFIT_E_INVALID, std::string("invalid ") + who + " arguments"};
+}
+}  // namespace
+
+extern "C" {
+
 int shorefit_lasso_forward(const ShorefitDesign* h, const float* signals, int64_t n_signals, const float* alpha,
                            double alpha_scalar, const float* warm, uint32_t max_iter, double tol,
                            int32_t non_negative, float* coef, uint32_t* iterations, void* stream) {
     return guarded([&] {
-        if (!h || (n_signals > 0 && (!signals || !coef)) || n_signals < 0)
-            throw Error{SHOREFIT_E_INVALID, "invalid lasso_forward arguments"};
+        check_lasso_args(h, signals, n_signals, coef, "lasso_forward");
         hipStream_t s = as_stream(stream);
         StreamScope scope(s);
-        const Design& d = h->d;
         const int64_t V = n_signals;
-        DevBuf<float> B(size_t(V) * d.Kp), mean(V);
-        FistaLaunch L{};
-        L.data = signals;
-        L.vox = nullptr;
-        L.n_vox = V;
-        L.N = d.N;
-        L.sig_stride = d.N;
-        L.K = d.K;
-        L.Kp = d.Kp;
-        L.Np = d.Np;
-        L.alpha = alpha_scalar;
-        L.max_iter = max_iter;
-        L.tol = float(tol);
-        L.nonneg = non_negative;
-        L.precision = SHOREFIT_PREC_FP32;
-        L.coef = coef;
-        L.iters = iterations;
-        L.A64p = h->ops.Ap.p;
-        L.c_dev = h->work.sc.p + 1;
+        FistaLaunch L = lasso_launch(h, signals, V, alpha, alpha_scalar, warm, max_iter, tol, non_negative, coef,
+                                     iterations);
+        DevBuf<float> B(size_t(V) * h->d.Kp), mean(V);
         L.Bbuf = B.p;
         L.meanbuf = mean.p;
-        L.Mfrag = h->ops.M.p;
-        L.Afrag = h->ops.A.p;
-        L.step = float(h->ops.step);
-        L.step64 = h->ops.step;
-        L.thr = float(h->ops.step * alpha_scalar);
-        L.alpha_vox = alpha;
-        L.warm = warm;
-        // persistent FP32 form: voxels of uneven convergence refill their slots
         DevBuf<float> mom;
         upload_momentum(mom, max_iter, s);
         DevBuf<unsigned int> counter(1);
         L.mom = mom.p;
         L.counter = counter.p;
-        L.persist = true;
+        launch_fista_rhs(L, s);
+        launch_fista(L, s);
+    });
+}
+
+int shorefit_lasso_forward_noalloc(const ShorefitDesign* h, const float* signals, int64_t n_signals,
+                                   const float* alpha, double alpha_scalar, const float* warm, uint32_t max_iter,
+                                   double tol, int32_t non_negative, float* coef, uint32_t* iterations,
+                                   void* stream) {
+    return guarded([&] {
+        check_lasso_args(h, signals, n_signals, coef, "lasso_forward_noalloc");
+        if (n_signals == 0) return;
+        const size_t V = size_t(n_signals), Kp = size_t(h->d.Kp);
+        const size_t n_mom = std::max<uint32_t>(max_iter, 1);
+        auto up = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+        const size_t off_mom = 256, off_B = off_mom + up(4 * n_mom), off_mean = off_B + up(4 * V * Kp);
+        const size_t need = off_mean + up(4 * V);
+        if (need > kNoallocScratchBytes) {
+            const size_t fixed = off_B, per = 4 * Kp + 4;
+            const size_t fit = kNoallocScratchBytes > fixed + 512 ? (kNoallocScratchBytes - fixed - 512) / per : 0;
+            throw Error{SHOREFIT_E_INVALID,
+                        "lasso_forward_noalloc: " + std::to_string(V) + " signals need " + std::to_string(need) +
+                            " bytes of device scratch; the library holds " + std::to_string(kNoallocScratchBytes) +
+                            " (at most " + std::to_string(fit) + " signals per call with this design and max_iter)"};
+        }
+        hipStream_t s = as_stream(stream);
+        char* base = static_cast<char*>(noalloc_scratch());
+        FistaLaunch L = lasso_launch(h, signals, int64_t(V), alpha, alpha_scalar, warm, max_iter, tol, non_negative,
+                                     coef, iterations);
+        float* mom = reinterpret_cast<float*>(base + off_mom);
+        L.counter = reinterpret_cast<unsigned int*>(base);
+        L.counter_zeroed = true;
+        L.mom = mom;
+        L.Bbuf = reinterpret_cast<float*>(base + off_B);
+        L.meanbuf = reinterpret_cast<float*>(base + off_mean);
+        launch_noalloc_prepare(mom, uint32_t(n_mom), L.counter, s);
         launch_fista_rhs(L, s);
         launch_fista(L, s);
     });

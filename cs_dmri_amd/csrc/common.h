@@ -187,6 +187,7 @@ struct FistaLaunch {
     // `mom`; with Xbuf / itsbuf / statusbuf / alphabuf the outputs go through
     // the epilogue kernel, without them the coefficients are written directly)
     bool persist;
+    bool counter_zeroed;    // the caller has cleared `counter` (no memset here)
     // alpha path strategies (path_n > 0): point i of voxel v is the FISTA
     // solve at alpha_i = alpha_max(v) path_f[i], warm-started from point i-1;
     // a selection pass then writes the chosen point's outputs.
@@ -204,6 +205,14 @@ struct FistaLaunch {
 void launch_fista_rhs(const FistaLaunch& L, hipStream_t s);
 // The FISTA solve proper (reads L.Bbuf / L.meanbuf).
 void launch_fista(const FistaLaunch& L, hipStream_t s);
+
+// The allocation-free lasso forward's device scratch: kNoallocScratchBytes of
+// a file-scope device array, one per device, shared by every call on it.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+// Its address on the current device (hipGetSymbolAddress on first use).
+void* noalloc_scratch();
+// Clears the work counter and writes the momentum table, both with kernels.
+void launch_noalloc_prepare(float* mom, uint32_t n_mom, unsigned int* counter, hipStream_t s);
 
 struct TikhonovLaunch {
     const float* data;

@@ -1501,7 +1501,7 @@ void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t
     if (per_cu < 1) per_cu = 1;
     const int64_t need = (args.n_vox + 16 * WAVES - 1) / (16 * WAVES);
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(need, int64_t(cus) * per_cu));
-    SF_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
+    if (!L.counter_zeroed) SF_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), s));
     hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(64 * WAVES), lds, s, args, counter, L.Xbuf, L.itsbuf,
                        L.statusbuf, L.alphabuf);
     SF_HIP(hipGetLastError());

@@ -217,6 +217,23 @@ int shorefit_lasso_forward(const ShorefitDesign* design, const float* signals, i
                            double alpha_scalar, const float* warm, uint32_t max_iter, double tol,
                            int32_t non_negative, float* coef, uint32_t* iterations, void* stream);
 
+/* shorefit_lasso_forward without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as shorefit_lasso_forward (bit for bit); the
+ * per-call buffers (right-hand sides, signal means, momentum table, work
+ * counter) live in a 64 MiB device array the library holds on each GPU until
+ * the process ends, so a call needs 4 (Kp + 1) bytes per signal plus
+ * 4 max_iter + 512 bytes (Kp = 160 at order 8: about 100k signals per call);
+ * larger calls return SHOREFIT_E_INVALID and name the limit. The call only
+ * queues work on `stream` (it returns before the results are ready). Every
+ * call on one GPU shares that array: calls that overlap on one GPU race, so
+ * callers serialise them. */
+int shorefit_lasso_forward_noalloc(const ShorefitDesign* design, const float* signals, int64_t n_signals,
+                                   const float* alpha, double alpha_scalar, const float* warm, uint32_t max_iter,
+                                   double tol, int32_t non_negative, float* coef, uint32_t* iterations,
+                                   void* stream);
+
 /* Vector-Jacobian product of the lasso solution by implicit differentiation
  * on each signal's active set S = {k : coef_k != 0}, z = sign(coef_S): with
  * G = A^T A, w = G_SS^-1 grad_coef_S (FP64 Cholesky), grad_signal = A_S w
