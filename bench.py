@@ -29,8 +29,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "voxel fits/s (SHORE-8 FISTA, coef err≤1e-4 vs FP64) + % MFMA/HBM roofline"
 PEAK = {"fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
-KERNEL = {"fp32": "fista_f32_persist_kernel<10,8,true>", "fp16": "fista_mixed_kernel<10,8,false,true,false,true,false>",
-          "bf16": "fista_mixed_kernel<10,8,true,true,false,false,false>"}
+KERNEL = {"fp32": "fista_f32_persist_kernel<10,8,true>",
+          "fp16": "fista_mixed_kernel<10,8,false,true,false,false,false,145>",
+          "bf16": "fista_mixed_kernel<10,8,true,true,false,false,false,145>"}
 # HBM bytes per launch of the solve kernel on this workload, from rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch), each scaled by
 # what the counter reads for a plain 1 GiB device copy (bench.py

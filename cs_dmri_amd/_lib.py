@@ -28,7 +28,7 @@ EXPORTS = (
     "shorefit_fit_graddev", "shorefit_graddev_variants", "shorefit_graddev_tune", "shorefit_graddev_gram",
     "shorefit_design_create", "shorefit_design_destroy", "shorefit_design_info", "shorefit_lasso_forward",
     "shorefit_lasso_adjoint", "shorefit_volume_to_voxel_major", "shorefit_volume_to_file_order",
-    "shorefit_lasso_forward_noalloc",
+    "shorefit_lasso_forward_noalloc", "shorefit_release_cache",
 )
 
 
@@ -131,6 +131,8 @@ def _load() -> C.CDLL:
     lib.shorefit_last_error.restype = C.c_char_p
     lib.shorefit_version.argtypes = []
     lib.shorefit_version.restype = C.c_char_p
+    lib.shorefit_release_cache.argtypes = []
+    lib.shorefit_release_cache.restype = C.c_int
     _vp = C.c_void_p
     lib.shorefit_tournier_matrix.argtypes = [C.c_uint32, C.c_double, C.c_uint32, _pd]
     lib.shorefit_tournier_matrix.restype = C.c_int

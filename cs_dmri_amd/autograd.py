@@ -19,6 +19,7 @@ affine solution map inside a piece).
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 import torch
@@ -121,7 +122,19 @@ class ShoreLasso(torch.autograd.Function):
         (coef,) = ctx.saved_tensors
         want_s = ctx.needs_input_grad[0]
         want_a = ctx.alpha_is_tensor and ctx.needs_input_grad[1]
-        gs, ga, _ = ctx.design.adjoint(coef, grad_coef, want_signal=want_s, want_alpha=want_a)
+        gs, ga, st = ctx.design.adjoint(coef, grad_coef, want_signal=want_s, want_alpha=want_a)
+        bad = st != 0
+        n_bad = int(bad.sum())
+        if n_bad:
+            # G_SS is singular (e.g. more active coefficients than signal
+            # samples): the active-set derivative does not exist there
+            warnings.warn(f"lasso backward: {n_bad} of {len(st)} signals have an active set whose Gram block "
+                          "A_S^T A_S is not positive definite; their gradients are set to zero", RuntimeWarning,
+                          stacklevel=2)
+            if gs is not None:
+                gs[bad] = 0.0
+            if ga is not None:
+                ga[bad] = 0.0
         return gs, ga, None, None, None, None
 
 

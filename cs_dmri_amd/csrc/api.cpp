@@ -54,8 +54,35 @@ size_t round_block(size_t bytes) {
     return (bytes + g - 1) / g * g;
 }
 // Cached (unused) bytes kept per thread beyond which free blocks go back to
-// the driver.
-constexpr size_t kCacheKeep = size_t(64) << 30;
+// the driver: 1/16 of the device's memory, and never more than 16 GiB, so
+// that a process sharing the device (torch's allocator) keeps its room.
+size_t cache_keep_bytes(int dev) {
+    static std::mutex mu;
+    static std::map<int, size_t> keep;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = keep.find(dev);
+    if (it != keep.end()) return it->second;
+    size_t free_b = 0, total_b = 0;
+    size_t k = size_t(1) << 30;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b > 0)
+        k = std::min(total_b / 16, size_t(16) << 30);
+    else
+        (void)hipGetLastError();
+    keep[dev] = k;
+    return k;
+}
+
+// Frees every cached block of the calling thread (all devices).
+void drop_cached_blocks(BlockCache& c) {
+    for (auto& dv : c.free) {
+        for (auto& kv : dv.second) {
+            c.size_of.erase(kv.second);
+            (void)hipFree(kv.second);
+        }
+        dv.second.clear();
+    }
+    c.cached_bytes = 0;
+}
 }  // namespace
 
 void* cache_alloc(size_t bytes) {
@@ -106,7 +133,7 @@ StreamScope::~StreamScope() {
     for (void* p : c.parked) {
         auto it = c.size_of.find(p);
         if (it == c.size_of.end()) continue;
-        if (c.cached_bytes + it->second.second > kCacheKeep) {
+        if (c.cached_bytes + it->second.second > cache_keep_bytes(it->second.first)) {
             (void)hipFree(p);
             c.size_of.erase(it);
             continue;
@@ -1033,6 +1060,15 @@ extern "C" {
 const char* shorefit_last_error(void) { return g_last_error.c_str(); }
 
 const char* shorefit_version(void) { return "shorefit 0.1.0 (gfx950)"; }
+
+int shorefit_release_cache(void) {
+    return guarded([&] {
+        BlockCache& c = block_cache();
+        if (c.size_of.empty()) return;
+        SF_HIP(hipDeviceSynchronize());
+        drop_cached_blocks(c);
+    });
+}
 
 int shorefit_n_coeffs(uint32_t radial_order) {
     if (radial_order % 2 != 0) return -1;

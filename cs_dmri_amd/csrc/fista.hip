@@ -949,9 +949,22 @@ __device__ __forceinline__ float pow2_to(float bound, int target) {
 // fit, the FP32 refresh moves fewer stopping iterations off the FP64 fit's
 // (92.8% equal against 89.5% at refresh 40) for 6% more solve time, so the
 // split refresh is kept only where the increments are not split.
-template <int MB, int WAVES, bool BF16, bool SPLITINC, bool NONNEG, bool SPLIT, bool PATH>
+// KL: the coefficient count when it is known at compile time (0: every padded
+// row is treated as live). Rows from KL to the padded count stay exactly zero
+// (zero rows of -cG and of b), so registers that hold only such rows are
+// skipped: at K = 145 in 160 rows, registers 1-3 of the last block; at K = 72
+// in 96, the whole last block, MFMAs included.
+template <int MB, int KL>
+struct LiveRows {
+    static constexpr int MBL = KL > 0 ? (KL + 15) / 16 : MB;                 // blocks with a real row
+    static constexpr int RL = KL > 0 ? (KL - 16 * (MBL - 1) < 4 ? KL - 16 * (MBL - 1) : 4) : 4;
+    static constexpr bool reg(int mb, int r) { return mb < MBL - 1 || (mb == MBL - 1 && r < RL); }
+};
+
+template <int MB, int WAVES, bool BF16, bool SPLITINC, bool NONNEG, bool SPLIT, bool PATH, int KL = 0>
 __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs a) {
     constexpr int KC = MB / 2;
+    using LV = LiveRows<MB, KL>;
     constexpr int NF = MB * KC * 64;  // uint4 fragments of one operator half
     extern __shared__ uint4 smem[];
     uint4* Ml = smem;  // lowp(-cG), then (SPLITINC) the remainder
@@ -965,6 +978,8 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
 
     floatx4 x[MB], y[MB], acc[MB], breg[MB];
     floatx4 x2[MB];  // the other register set of x in the inner loop
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) x2[mb] = floatx4{0.f, 0.f, 0.f, 0.f};  // its dead rows stay zero
     int64_t vid = -1;
     uint32_t it = 0;
     float thr_u = a.thr, thr = a.thr, sc = 1.f, mprev = 0.f, dprev = 0.f, mom = 0.f;
@@ -1038,7 +1053,8 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) q = fmaf(xv[mb][r], xv[mb][r], q);
+            for (int r = 0; r < 4; ++r)
+                if (LV::reg(mb, r)) q = fmaf(xv[mb][r], xv[mb][r], q);
         return sqrtf(lane_sum4(q));
     };
 
@@ -1100,6 +1116,10 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int mb = 2 * kc + s2;
+                        if (!LV::reg(mb, r)) {  // padding rows: x, y, acc and dy are zero
+                            t[4 * s2 + r] = 0.f;
+                            continue;
+                        }
                         const float val = y[mb][r] + acc[mb][r];
                         float xn = val - __builtin_amdgcn_fmed3f(val, -thr, thr);  // strict soft threshold
                         if constexpr (NONNEG) xn = fmaxf(xn, 0.f);
@@ -1278,15 +1298,16 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                     }
                 } else {
 #pragma unroll
-                    for (int mb = 0; mb < MB; ++mb) {
+                    for (int mb = 0; mb < LV::MBL; ++mb) {  // padding blocks keep acc = 0
                         floatx4 c = SPLITINC ? acc[mb] : breg[mb];
                         c *= sc;
 #pragma unroll
-                        for (int kb = 0; kb < MB; ++kb) {
+                        for (int kb = 0; kb < LV::MBL; ++kb) {
                             const floatx4 mf = *reinterpret_cast<const floatx4*>(
                                 reinterpret_cast<const char*>(a.M32) + (loff + uint32_t(mb * MB + kb) * 1024u));
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) c = mfma4(mf[r], y[kb][r], c);
+                            for (int r = 0; r < 4; ++r)
+                                if (LV::reg(kb, r)) c = mfma4(mf[r], y[kb][r], c);
                         }
                         acc[mb] = c;
                     }
@@ -1297,12 +1318,12 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                 // fragments in flight from LDS while one block's MFMAs run.
                 uint4 fr[2 * MB * KC];
 #pragma unroll
-                for (int i = 0; i < MB * KC; ++i) {
+                for (int i = 0; i < LV::MBL * KC; ++i) {
                     fr[2 * i] = Ml[moff + i * 64 + lane];
                     fr[2 * i + 1] = Ml[moff + NF + i * 64 + lane];
                 }
 #pragma unroll
-                for (int mb = 0; mb < MB; ++mb) {
+                for (int mb = 0; mb < LV::MBL; ++mb) {
                     // the chain starts from zero, so its roundings are relative
                     // to the increment, not to acc; one rounding adds it
                     floatx4 c = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -1312,13 +1333,15 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
                         c = mfma16<BF16>(fr[2 * (mb * KC + kc)], dv, c);
                         c = mfma16<BF16>(fr[2 * (mb * KC + kc) + 1], dv, c);
                     }
-                    acc[mb] += c;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (LV::reg(mb, r)) acc[mb][r] += c[r];
                 }
                 __builtin_amdgcn_sched_group_barrier(0x100, 2 * KC, 0);  // DS reads
 #pragma unroll
-                for (int mb = 0; mb < MB; ++mb) {
+                for (int mb = 0; mb < LV::MBL; ++mb) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 2 * KC, 0);  // MFMA
-                    if (mb + 1 < MB) __builtin_amdgcn_sched_group_barrier(0x100, 2 * KC, 0);
+                    if (mb + 1 < LV::MBL) __builtin_amdgcn_sched_group_barrier(0x100, 2 * KC, 0);
                 }
             } else {
                 // acc += (-cG) fp16(dy). The schedule keeps the fragments of
@@ -1556,10 +1579,17 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     // refresh on the FP32 MFMA, except FP16 without split increments (order
     // 10), which refreshes from FP16 hi/lo splits
     constexpr bool SPLIT = !BF16 && !SPLITINC;
-    auto kern = path ? (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, true, SPLIT, true>
-                                 : fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, false, SPLIT, true>)
-                     : (L.nonneg ? fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, true, SPLIT, false>
-                                 : fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, false, SPLIT, false>);
+    // the coefficient counts of radial orders 6 and 8 are compiled in
+    constexpr int KL = MB == 10 ? 145 : (MB == 6 ? 72 : 0);
+    const bool kl = KL > 0 && L.K == KL;
+    auto pick = [&](auto nonneg, auto pth) {
+        constexpr bool NN = decltype(nonneg)::value, PT = decltype(pth)::value;
+        return kl ? fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, NN, SPLIT, PT, KL>
+                  : fista_mixed_kernel<MB, WAVES, BF16, SPLITINC, NN, SPLIT, PT, 0>;
+    };
+    auto kern = path ? (L.nonneg ? pick(std::true_type{}, std::true_type{}) : pick(std::false_type{}, std::true_type{}))
+                     : (L.nonneg ? pick(std::true_type{}, std::false_type{})
+                                 : pick(std::false_type{}, std::false_type{}));
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               int(lds));
     int dev = 0, cus = 0, per_cu = 0;

@@ -425,13 +425,8 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
     }
 }
 
-// Column n of P = (L L^T)^-1 A^T is the solution for right-hand side row n of
-// A (forward then backward substitution), computed in place in P's column
-// (P is K x N row-major, so threads of consecutive n touch consecutive words).
-// P = (L L^T)^-1 A^T for CB columns per workgroup: L (packed lower triangle)
-// and the workgroup's CB columns in LDS, 16 threads per column summing
-// strided partial dot products, combined by a butterfly.
-// P = L^-T L^-1 A^T, 16 columns of P per workgroup. The 16 lanes of a column
+// P = L^-T L^-1 A^T, 16 columns of P per workgroup (column n solves for row n
+// of A; P is K x N row-major). The 16 lanes of a column
 // (all in one wave) keep its vector in registers, lane u the entries
 // u + 16 r; a substitution step divides the solved entry on its lane, hands it
 // to the other 15 with a 16-lane shuffle and lets each lane update its own

@@ -15,6 +15,12 @@ from . import _lib as L
 TORTOISE_DEFAULT_GMAX = 80e-3
 
 
+def release_device_cache() -> None:
+    """Return the library's cached device blocks of this thread to the driver
+    (e.g. when torch's allocator runs short of memory)."""
+    L.check(L.lib().shorefit_release_cache())
+
+
 def n_coeffs(radial_order: int) -> int:
     k = L.lib().shorefit_n_coeffs(int(radial_order))
     if k < 0:

@@ -240,7 +240,7 @@ int shorefit_lasso_forward_noalloc(const ShorefitDesign* design, const float* si
  * (V x N) and grad_alpha = -N z^T w (V). Device pointers; grad_signal and
  * grad_alpha may be NULL; status (V, may be NULL) is 0, or 1 when G_SS is not
  * positive definite (the signal's gradients are then NaN). Active sets of at
- * most 190 coefficients. */
+ * most 192 coefficients. */
 int shorefit_lasso_adjoint(const ShorefitDesign* design, const float* coef, const float* grad_coef,
                            int64_t n_signals, float* grad_signal, float* grad_alpha, uint8_t* status, void* stream);
 
@@ -334,6 +334,13 @@ const char* shorefit_last_error(void);
 
 /* Library version string. */
 const char* shorefit_version(void);
+
+/* Returns the calling thread's cached device blocks to the driver (after a
+ * device synchronization). Entry points keep the blocks they release for
+ * later calls on the same thread, up to 1/16 of the device's memory (at most
+ * 16 GiB); a process that shares the device with another allocator (torch)
+ * calls this when that allocator runs short or at teardown. */
+int shorefit_release_cache(void);
 
 #ifdef __cplusplus
 }

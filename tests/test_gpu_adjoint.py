@@ -199,6 +199,33 @@ def test_autograd_function(problem):
     assert its.dtype == torch.int32 and not its.requires_grad
 
 
+def test_autograd_singular_active_set():
+    # A zero column of A whose coefficient is active (a warm start keeps it
+    # away from zero for a few iterations) makes G_SS singular: its pivot is
+    # exactly zero. The adjoint reports status 1 for those signals; the
+    # autograd backward warns and returns zero gradients for them and leaves
+    # the other signals' gradients as the adjoint gives them.
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((30, 10)) / np.sqrt(30)
+    A[:, 4] = 0.0
+    d = LassoDesign(A)
+    S_np = (A @ rng.standard_normal((10, 8))).T.astype(np.float32)
+    warm = torch.zeros((8, 10), device=DEV)
+    warm[:4, 4] = 100.0  # signals 0-3: column 4 active
+    S = torch.from_numpy(S_np).to(DEV).requires_grad_(True)
+    alpha = torch.full((8,), 1e-3, device=DEV, requires_grad=True)
+    coef, _ = lasso(S, alpha, d, max_iter=3, tol=0.0, warm=warm)
+    assert bool((coef[:4, 4] != 0).all()) and not coef[4:, 4].any()
+    g = torch.ones_like(coef)
+    gs, ga, st = d.adjoint(coef.detach(), g)
+    assert st.tolist() == [1] * 4 + [0] * 4
+    with pytest.warns(RuntimeWarning, match="not positive definite"):
+        (coef * g).sum().backward()
+    assert not S.grad[:4].any() and not alpha.grad[:4].any()
+    assert torch.equal(S.grad[4:], gs[4:]) and torch.equal(alpha.grad[4:], ga[4:])
+    assert torch.isfinite(S.grad).all() and torch.isfinite(alpha.grad).all()
+
+
 def test_hypergradient_vs_grid(problem):
     A_tr, A_te, S_tr, S_te = problem
     d = LassoDesign(A_tr)
