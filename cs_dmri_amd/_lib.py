@@ -28,8 +28,10 @@ EXPORTS = (
     "shorefit_fit_graddev", "shorefit_graddev_variants", "shorefit_graddev_tune", "shorefit_graddev_gram",
     "shorefit_design_create", "shorefit_design_destroy", "shorefit_design_info", "shorefit_lasso_forward",
     "shorefit_lasso_adjoint", "shorefit_volume_to_voxel_major", "shorefit_volume_to_file_order",
-    "shorefit_lasso_forward_noalloc", "shorefit_release_cache",
+    "shorefit_lasso_forward_noalloc", "shorefit_release_cache", "shorefit_build_info", "shorefit_set_progress",
 )
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.c_void_p)
 
 
 class ShorefitSpec(C.Structure):
@@ -133,6 +135,10 @@ def _load() -> C.CDLL:
     lib.shorefit_version.restype = C.c_char_p
     lib.shorefit_release_cache.argtypes = []
     lib.shorefit_release_cache.restype = C.c_int
+    lib.shorefit_build_info.argtypes = []
+    lib.shorefit_build_info.restype = C.c_char_p
+    lib.shorefit_set_progress.argtypes = [PROGRESS_FN, C.c_void_p, C.c_double]
+    lib.shorefit_set_progress.restype = C.c_int
     _vp = C.c_void_p
     lib.shorefit_tournier_matrix.argtypes = [C.c_uint32, C.c_double, C.c_uint32, _pd]
     lib.shorefit_tournier_matrix.restype = C.c_int

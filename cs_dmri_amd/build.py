@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+import time
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -30,11 +31,31 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build libshorefit.so)")
 
 
-def flags() -> list[str]:
+def build_stamp() -> tuple[str, str]:
+    """(git SHA of the sources, with "-dirty" for uncommitted changes, or
+    "unknown"; UTC build time in ISO 8601) baked into the library for the
+    provenance record of its outputs."""
+    sha = "unknown"
+    try:
+        r = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "HEAD"], capture_output=True, text=True, timeout=10)
+        if r.returncode == 0 and r.stdout.strip():
+            sha = r.stdout.strip()
+            d = subprocess.run(["git", "-C", str(ROOT), "status", "--porcelain", "--untracked-files=no"],
+                               capture_output=True, text=True, timeout=10)
+            if d.returncode == 0 and d.stdout.strip():
+                sha += "-dirty"
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return sha, time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime())
+
+
+def flags(stamp: tuple[str, str] | None = None) -> list[str]:
+    sha, ts = stamp or ("unknown", "unknown")
     return [
         f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         f"-I{ROOT / 'include'}", f"-I{CSRC}",
+        f'-DSHOREFIT_GIT_SHA="{sha}"', f'-DSHOREFIT_BUILD_TIME="{ts}"',
     ]
 
 
@@ -44,12 +65,13 @@ def build(verbose: bool = False) -> Path:
     objdir.mkdir(exist_ok=True)
     objs = []
     procs = []
+    stamp = build_stamp()
     for src in SOURCES:
         path = CSRC / src
         if not path.exists():
             continue
         obj = objdir / (path.stem + ".o")
-        cmd = [cc, *flags(), "-x", "hip", "-c", str(path), "-o", str(obj)]
+        cmd = [cc, *flags(stamp), "-x", "hip", "-c", str(path), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

@@ -2,6 +2,7 @@
 // (src/fit.rs:545-805), gradient-table handling (src/qspace.rs:90-227), and
 // the device set-up / fit launches.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -11,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -283,6 +285,57 @@ struct FistaWork {
     DevBuf<uint32_t> its, itspath;
     DevBuf<uint8_t> status, convpath;
 };
+
+// Progress reporting (the reference's on_voxel callback behind cs-fit's
+// heartbeat, src/fit.rs:69-73, src/progress.rs:17-80): a callback registered
+// per thread, polled from the calling thread while the persistent solve
+// kernel runs. The kernel publishes each wave's finished-voxel count into a
+// host-pinned array; nothing is called from the device.
+struct ProgressReg {
+    shorefit_progress_fn fn = nullptr;
+    void* user = nullptr;
+    double interval = 1.0;
+};
+ProgressReg& progress_reg() {
+    thread_local ProgressReg r;
+    return r;
+}
+constexpr int kProgressSlots = 1 << 16;  // waves of the largest solve grid
+unsigned int* progress_slots_pinned() {
+    struct Pinned {
+        unsigned int* p = nullptr;
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    thread_local Pinned h;
+    if (!h.p)
+        SF_HIP(hipHostMalloc(reinterpret_cast<void**>(&h.p), sizeof(unsigned int) * kProgressSlots,
+                             hipHostMallocCoherent));
+    return h.p;
+}
+// Calls the registered callback until `solve_done` has completed, every
+// interval seconds, with the sum of the published counts.
+void poll_progress(const unsigned int* slots, int64_t total, hipEvent_t solve_done) {
+    const ProgressReg& r = progress_reg();
+    auto last = std::chrono::steady_clock::now();
+    const auto step = std::chrono::duration<double>(std::max(r.interval, 0.001));
+    for (;;) {
+        const hipError_t q = hipEventQuery(solve_done);
+        if (q != hipErrorNotReady) {
+            if (q != hipSuccess) SF_HIP(q);
+            break;
+        }
+        std::this_thread::sleep_for(std::min(std::chrono::duration<double>(0.005), step));
+        const auto now = std::chrono::steady_clock::now();
+        if (now - last < step) continue;
+        last = now;
+        int64_t done = 0;
+        for (int i = 0; i < kProgressSlots; ++i)
+            done += reinterpret_cast<const volatile unsigned int*>(slots)[i];
+        r.fn(std::min(done, total), total, r.user);
+    }
+}
 
 // A small pinned host buffer per thread for device scalars read mid-call.
 double* pinned_scalars() {
@@ -950,8 +1003,16 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
         }
         SF_HIP(hipEventRecord(e0, s));
         L.event_mid = e1;  // recorded right after the solve kernel
+        const bool report = progress_reg().fn && V_shared > 0;
+        if (report) {
+            L.progress = progress_slots_pinned();
+            L.progress_slots = kProgressSlots;
+            std::memset(L.progress, 0, sizeof(unsigned int) * kProgressSlots);
+            progress_reg().fn(0, V, progress_reg().user);
+        }
         launch_fista(L, s);
         if (V_shared == 0) SF_HIP(hipEventRecord(e1, s));
+        if (report) poll_progress(L.progress, V_shared, e1);
         SF_HIP(hipEventRecord(e2, s));
         if (gd && V_dv > 0) {
             // deviated voxels: per-voxel designs (src/graddev.rs:160-185)
@@ -984,6 +1045,7 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
         L.r2 = r2;
         L.iters = its;
         L.kind = kind;
+        if (progress_reg().fn) progress_reg().fn(0, V, progress_reg().user);
         SF_HIP(hipEventRecord(e0, s));
         launch_tikhonov(L, s);
         SF_HIP(hipEventRecord(e1, s));
@@ -991,6 +1053,7 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
         SF_HIP(hipEventRecord(e3, s));
     }
     SF_HIP(hipStreamSynchronize(s));
+    if (progress_reg().fn) progress_reg().fn(V, V, progress_reg().user);
     float ms = 0.f, ms_gd = 0.f;
     SF_HIP(hipEventElapsedTime(&ms, e0, e1));
     SF_HIP(hipEventElapsedTime(&ms_gd, e2, e3));
@@ -1060,6 +1123,29 @@ extern "C" {
 const char* shorefit_last_error(void) { return g_last_error.c_str(); }
 
 const char* shorefit_version(void) { return "shorefit 0.1.0 (gfx950)"; }
+
+#ifndef SHOREFIT_GIT_SHA
+#define SHOREFIT_GIT_SHA "unknown"
+#endif
+#ifndef SHOREFIT_BUILD_TIME
+#define SHOREFIT_BUILD_TIME "unknown"
+#endif
+
+const char* shorefit_build_info(void) {
+    return "{\"library\": \"shorefit\", \"version\": \"0.1.0\", \"git_sha\": \"" SHOREFIT_GIT_SHA
+           "\", \"build_timestamp\": \"" SHOREFIT_BUILD_TIME "\", \"offload_arch\": \"gfx950\"}";
+}
+
+int shorefit_set_progress(shorefit_progress_fn fn, void* user, double interval_seconds) {
+    return guarded([&] {
+        if (fn && !(interval_seconds > 0.0))
+            throw Error{SHOREFIT_E_INVALID, "progress interval must be positive"};
+        ProgressReg& r = progress_reg();
+        r.fn = fn;
+        r.user = user;
+        r.interval = interval_seconds;
+    });
+}
 
 int shorefit_release_cache(void) {
     return guarded([&] {

@@ -200,6 +200,10 @@ struct FistaLaunch {
     float* Xpath;           // [n_vox][path_n][Kp]
     uint32_t* itspath;      // [n_vox][path_n]
     uint8_t* convpath;      // [n_vox][path_n]
+    // progress of the persistent solve kernels (optional): host-pinned,
+    // device-visible, one count of finished voxels per wave of the grid
+    unsigned int* progress;
+    int progress_slots;
 };
 // b = c A^T s and signal means into L.Bbuf / L.meanbuf (FP64 accumulation).
 void launch_fista_rhs(const FistaLaunch& L, hipStream_t s);

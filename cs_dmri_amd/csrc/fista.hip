@@ -94,7 +94,19 @@ struct FistaArgs {
     int path_rule;             // 0: BIC argmin, 1: L2-anchored
     double slack;
     const float* rss_l2_ref;   // L2-anchored: Tikhonov RSS, indexed like the outputs
+    unsigned int* progress;    // host-pinned finished-voxel count per wave (may be null)
+    int progress_slots;
 };
+
+// Publishes a wave's count of finished voxels to the host-pinned progress
+// array the host polls while the solve runs: a system-scope (vector) store,
+// so the value reaches host memory without waiting for the kernel's end.
+__device__ __forceinline__ void publish_progress(unsigned int* progress, int slots, int waves, unsigned int done) {
+    if (!progress) return;
+    const int slot = int(blockIdx.x) * waves + int(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0 && slot < slots)
+        __hip_atomic_store(progress + slot, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Prologue: b = c A^T s for the wave's 16 voxels (c = 1/(N L) folded into
 // ATfrag). Signal chunk c of 16 samples: lane (h, j) loads s_j[16c + 4h + r].
@@ -638,6 +650,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
     bool active = false, zero_voxel = false;
     bool need = true;  // this slot (all four lanes of column j agree) wants a voxel
     float bmax = 0.f;
+    unsigned int wdone_lane = 0;  // voxels this slot has finished (on each of its four lanes)
     int pi = 0;        // alpha-path point of the slot (PATH)
 
     // outputs of the slot's voxel (src/solver/fista.rs:113-143 diagnostics):
@@ -672,6 +685,12 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
 
     for (;;) {
         if (__ballot(need) != 0ull) {
+            if (a.progress) {
+                unsigned int tot = h == 0 ? wdone_lane : 0u;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o);
+                publish_progress(a.progress, a.progress_slots, WAVES, tot);
+            }
             // slots without a voxel take the next one (lane j of the column
             // draws it, the other three read it from lane j)
             unsigned int nv = 0;
@@ -728,6 +747,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
                     }
                 its = 0;
                 if (PATH && zero_voxel) {
+                    ++wdone_lane;
                     // every path point of an all-zero voxel: 0 iterations, status 3
                     if (h == 0)
                         for (int i = 0; i < a.path_n; ++i) {
@@ -735,6 +755,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
                             a.convpath[v * a.path_n + i] = 3;
                         }
                 } else if (!PATH && (zero_voxel || a.max_iter == 0)) {
+                    ++wdone_lane;
                     finish(zero_voxel);  // nothing to iterate: take another voxel next round
                 } else {
                     active = true;
@@ -801,10 +822,12 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
                         its = 0;
                         next = true;
                     } else {
+                        ++wdone_lane;
                         active = false;
                         need = true;
                     }
                 } else {
+                    ++wdone_lane;
                     finish(conv);
                     active = false;
                     need = true;
@@ -895,6 +918,8 @@ struct MixedArgs {
     float* Xpath;
     uint32_t* itspath;
     uint8_t* convpath;
+    unsigned int* progress;  // host-pinned finished-voxel count per wave (may be null)
+    int progress_slots;
 };
 
 template <bool BF16>
@@ -1083,6 +1108,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     };
 
     uint32_t wit = 0;
+    unsigned int wdone = 0;  // voxels this wave has finished
     const float tol = a.tol;
     // Outer loop: write out finished slots and refill them. Inner loop:
     // iterate until some slot finishes. Keeping the refill (which rewrites
@@ -1386,6 +1412,8 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
             if (step(x2, x, std::false_type{})) break;
         }
         if (done && !wrote) write_out(x, conv);
+        wdone += unsigned(__popcll(__ballot(done && h == 0)));
+        publish_progress(a.progress, a.progress_slots, WAVES, wdone);
         refill(done);
     }
 }
@@ -1574,6 +1602,8 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
     m.Xpath = L.Xpath;
     m.itspath = L.itspath;
     m.convpath = L.convpath;
+    m.progress = L.progress;
+    m.progress_slots = L.progress_slots;
     const bool path = L.path_n > 0;
     const size_t lds = size_t(MB) * KC * 64 * 16 * (SPLITINC ? 2 : 1);
     // refresh on the FP32 MFMA, except FP16 without split increments (order
@@ -1673,6 +1703,8 @@ void launch_fista(const FistaLaunch& L, hipStream_t s) {
     a.path_rule = L.path_rule;
     a.slack = L.slack;
     a.rss_l2_ref = L.rss_l2_ref;
+    a.progress = L.progress;
+    a.progress_slots = L.progress_slots;
     if (L.precision != SHOREFIT_PREC_FP32) {
         const bool bf = L.precision == SHOREFIT_PREC_BF16;
         switch (L.Kp / 16) {

@@ -27,6 +27,8 @@ from __future__ import annotations
 
 import json
 import os
+import socket
+import sys
 import time
 from dataclasses import dataclass, field
 from pathlib import Path
@@ -406,22 +408,190 @@ def sparsity_map(coeffs: np.ndarray, mask: np.ndarray) -> np.ndarray:
     return np.where(mask, nz / np.float32(coeffs.shape[-1]), np.float32(0.0)).astype(np.float32)
 
 
+# ------------------------------------------------------------ the cs-fit path --
+def ensure_nifti_extension(path, label: str = "--output") -> Path:
+    """``path`` itself when it ends in .nii or .nii.gz, else ``path`` + .nii.gz
+    with a warning on stderr (src/io/output.rs:28-50): the writer picks the
+    format from the extension."""
+    s = str(path)
+    if s.endswith(".nii.gz") or s.endswith(".nii"):
+        return Path(path)
+    new = Path(s + ".nii.gz")
+    print(f"[cs_dmri] warning: {label} {s} has no .nii.gz/.nii extension; appending .nii.gz \u2192 {new}",
+          file=sys.stderr)
+    return new
+
+
+def precheck_writable(path, overwrite: bool) -> None:
+    """Refuses, before any work, an output that exists (without ``overwrite``)
+    or whose directory cannot be created or written: the parent directory is
+    created if missing and probed with a short-lived file
+    (src/io/output.rs:52-103)."""
+    path = Path(path)
+    if path.exists() and not overwrite:
+        raise FileExistsError(f"refusing to overwrite existing {path} \u2014 pass overwrite=True to replace")
+    parent = path.parent if str(path.parent) not in ("", ".") else Path(".")
+    try:
+        parent.mkdir(parents=True, exist_ok=True)
+    except OSError as e:
+        raise PermissionError(f"cannot create parent directory {parent} for output {path}: {e}") from None
+    probe = parent / f".cs_dmri-precheck.{os.getpid()}.{time.time_ns() % 1_000_000_000}"
+    try:
+        with open(probe, "wb") as f:
+            f.write(b"x")
+    except OSError as e:
+        raise PermissionError(f"output directory {parent} is not writable (probe failed): {e}") from None
+    finally:
+        try:
+            probe.unlink()
+        except FileNotFoundError:
+            pass
+
+
+def fit_output_paths(output, *, diagnostics=False, regularization="l1", alpha_mode="l2-anchored") -> list[Path]:
+    """Every file ``fit_files`` writes for these settings: the coefficient
+    file, its sidecar and, with diagnostics, the maps the fit produces
+    (``_r2``, ``_rmse``, ``_sparsity``; ``_alpha`` and ``_bic`` for L1;
+    ``_rss_l2`` for the L2-anchored strategy)."""
+    output = Path(output)
+    paths = [output, sidecar_path_for(output)]
+    if diagnostics:
+        suffixes = ["_r2", "_rmse", "_sparsity"]
+        if regularization == "l1":
+            suffixes += ["_alpha", "_bic"]
+            if alpha_mode == "l2-anchored":
+                suffixes.append("_rss_l2")
+        paths += [sibling_path(output, f"{sfx}.nii.gz") for sfx in suffixes]
+    return paths
+
+
+def precheck_outputs(output, overwrite: bool, **fit_kwargs) -> Path:
+    """cs-fit's pre-flight (src/bin/cs-fit.rs:507-552): the coefficient path
+    gets a NIfTI extension, then every output it will write is checked, so a
+    path mistake surfaces before the data are read. Returns the output path."""
+    output = ensure_nifti_extension(output)
+    for p in fit_output_paths(output, diagnostics=bool(fit_kwargs.get("diagnostics")),
+                              regularization=fit_kwargs.get("regularization", "l1"),
+                              alpha_mode=fit_kwargs.get("alpha_mode", "l2-anchored")):
+        precheck_writable(p, overwrite)
+    return output
+
+
+PROVENANCE_MODES = ("minimal", "full", "none")
+
+
+class ProvenanceRecorder:
+    """The sidecar's provenance record (src/io/provenance.rs:34-105): tool,
+    library version, git SHA and build time of the library, host threads and
+    wall time of the run; ``full`` adds the command line, host name and start
+    time, which may identify a subject and are left out by default."""
+
+    def __init__(self, tool: str = "cs-fit", mode: str = "minimal"):
+        if mode not in PROVENANCE_MODES:
+            raise ValueError(f"provenance must be one of {PROVENANCE_MODES}, got {mode!r}")
+        self.tool, self.mode = tool, mode
+        self._t0 = time.perf_counter()
+        self._started = time.strftime("%Y-%m-%dT%H:%M:%SZ", time.gmtime())
+
+    def finish(self, threads_used: int = 1) -> dict | None:
+        if self.mode == "none":
+            return None
+        info = json.loads(L.lib().shorefit_build_info().decode())
+        rec = {"tool": self.tool, "version": info["version"], "git_sha": info["git_sha"],
+               "build_timestamp": info["build_timestamp"], "threads_used": int(threads_used),
+               "duration_secs": time.perf_counter() - self._t0}
+        if self.mode == "full":
+            rec["argv"] = list(sys.argv)
+            rec["hostname"] = os.environ.get("HOSTNAME") or socket.gethostname()
+            rec["started_at_utc"] = self._started
+        return rec
+
+
+def format_hms(seconds: float) -> str:
+    """Whole seconds as HH:MM:SS (src/progress.rs:137-143)."""
+    t = int(max(seconds, 0.0))
+    return f"{t // 3600:02d}:{(t % 3600) // 60:02d}:{t % 60:02d}"
+
+
+class Heartbeat:
+    """Plain-line progress on stderr for a voxel loop (src/progress.rs:17-143).
+
+    ``update(done, total)`` is the progress callback of ``shore_fit``; it
+    prints ``[label] done/total (pct%) elapsed=HH:MM:SS eta=HH:MM:SS`` at most
+    once per ``interval`` seconds, and ``finish()`` prints the ``done`` line.
+    ``quiet`` prints nothing. No carriage returns, so job logs stay readable.
+    """
+
+    def __init__(self, label: str, total: int, interval: float, quiet: bool = False, stream=None):
+        if not interval > 0:
+            raise ValueError(f"interval must be positive, got {interval}")
+        self.label = label
+        self.total = int(total)
+        self.interval = float(interval)
+        self.quiet = bool(quiet)
+        self.done = 0
+        self.stream = stream
+        self._t0 = time.perf_counter()
+        self._next = self._t0 + self.interval
+
+    def _pct(self, done):
+        return 100.0 if self.total == 0 else done * 100.0 / self.total
+
+    def _eta(self, done, elapsed):
+        if done == 0 or done >= self.total:
+            return "--:--:--"
+        return format_hms((self.total - done) * elapsed / done)
+
+    def _print(self, line):
+        print(line, file=self.stream if self.stream is not None else sys.stderr, flush=True)
+
+    def update(self, done: int, total: int | None = None) -> None:
+        self.done = max(self.done, int(done))
+        now = time.perf_counter()
+        if self.quiet or self.total == 0 or now < self._next:
+            return
+        self._next = now + self.interval
+        el = now - self._t0
+        self._print(f"[{self.label}] {self.done}/{self.total} ({self._pct(self.done):.1f}%) "
+                    f"elapsed={format_hms(el)} eta={self._eta(self.done, el)}")
+
+    def finish(self) -> None:
+        if not self.quiet:
+            el = time.perf_counter() - self._t0
+            self._print(f"[{self.label}] done {self.done}/{self.total} ({self._pct(self.done):.1f}%) "
+                        f"elapsed={format_hms(el)}")
+
+
 def fit_files(dwi, bval, bvec, output, *, mask=None, overwrite=False, bvec_frame="world-ras", big_delta=None,
-              small_delta=None, device=None, **fit_kwargs) -> dict:
-    """The ``cs-fit`` data path (src/bin/cs-fit.rs:204-434): load the series
-    (``load_dwi``), fit with ``shore_fit`` (keywords as there; ``precision``
-    and ``refresh_every`` select the device arithmetic), write the
-    coefficient NIfTI and sidecar, and with ``diagnostics=True`` the
-    ``_r2``, ``_rmse``, ``_alpha``, ``_bic``, ``_rss_l2`` and ``_sparsity``
-    maps beside it. Returns the sidecar dict. With ``device`` the series is
-    converted and fitted without a host copy of the voxel-major volume."""
+              small_delta=None, device=None, provenance="minimal", quiet=False, progress_interval_secs=30,
+              **fit_kwargs) -> dict:
+    """The ``cs-fit`` data path (src/bin/cs-fit.rs:204-434): check every
+    output path first (``precheck_outputs``), load the series (``load_dwi``),
+    fit with ``shore_fit`` (keywords as there; ``precision`` and
+    ``refresh_every`` select the device arithmetic), write the coefficient
+    NIfTI and sidecar (with the ``provenance`` record: "minimal", "full" or
+    "none"), reporting progress on stderr every ``progress_interval_secs``
+    (at least 1 s; ``quiet`` silences it, src/bin/cs-fit.rs:175-183,
+    :285-312), and with ``diagnostics=True`` the ``_r2``, ``_rmse``, ``_alpha``,
+    ``_bic``, ``_rss_l2`` and ``_sparsity`` maps beside it. Returns the
+    sidecar dict. With ``device`` the series is converted and fitted without
+    a host copy of the voxel-major volume."""
     from .models import shore_fit
 
+    rec = ProvenanceRecorder("cs-fit", provenance)
+    output = precheck_outputs(output, overwrite, **fit_kwargs)
     d = load_dwi(dwi, bval, bvec, mask, big_delta=big_delta, small_delta=small_delta, bvec_frame=bvec_frame,
                  device=device)
+    n_mask = int((d.mask != 0).sum())
+    hb = Heartbeat("cs-fit", n_mask, max(float(progress_interval_secs), 1.0), quiet)
     out = shore_fit(d.data, d.bvals, d.bvecs, d.mask, big_delta=big_delta, small_delta=small_delta,
-                    bvec_frame=bvec_frame, **fit_kwargs)
-    sidecar = out["sidecar"]
+                    bvec_frame=bvec_frame, progress=None if quiet else hb.update,
+                    progress_interval=hb.interval, **fit_kwargs)
+    hb.finish()
+    sidecar = dict(out["sidecar"])
+    prov = rec.finish(threads_used=1)
+    if prov is not None:
+        sidecar["provenance"] = prov
     coeffs = out["coefficients"]
     write_coefficients(output, coeffs, dwi, sidecar, overwrite=overwrite)
     if fit_kwargs.get("diagnostics"):

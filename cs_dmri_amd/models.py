@@ -6,6 +6,7 @@ tests drive (FistaSolver / TikhonovSolver / ShoreBasis::design_matrix).
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -155,7 +156,7 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
               non_negative=False, lambda_n=1e-8, lambda_l=1e-8, nonneg_max_iter=200, nonneg_tol=1e-9,
               nonneg_epsilon=1e-10, diagnostics=False, allow_single_shell=False, n_threads=None,
               precision="fp32", refresh_every=None, graddev=None, bvecs_vox=None, frame_rotation=None,
-              graddev_variant=-1) -> dict:
+              graddev_variant=-1, progress=None, progress_interval=1.0) -> dict:
     """Drop-in for ``_cs_dmri.shore_fit`` (python/src/models.rs:70-198).
 
     Same keywords, defaults, validation messages (``ValueError``) and returned
@@ -172,6 +173,11 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
     (src/graddev.rs:160-185). ``bvecs_vox`` are the voxel-axis directions
     (default ``bvecs``) and ``frame_rotation`` R maps them to ``bvecs``
     (default identity). L1 with ``alpha_mode`` fixed or alpha-ratio only.
+
+    ``progress(done, total)`` is called with the number of masked voxels
+    finished: 0 when the solve starts, every ``progress_interval`` seconds
+    while it runs, and ``total`` at the end (the ``on_voxel`` hook of
+    src/fit.rs:69-73 behind cs-fit's heartbeat, polled on this thread).
     """
     if bvec_frame not in ("image-axis", "world-ras"):
         raise ValueError(f"bvec_frame must be 'image-axis' or 'world-ras', got {bvec_frame!r}")
@@ -235,15 +241,16 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
         setattr(out, name, None if arr is None else addr(arr))
     v = L.ShorefitVolume(addr(vol), addr(m), nx, ny, nz, 1 if on_dev else 0, 0)
     stream = torch.cuda.current_stream(dev).cuda_stream if on_dev else None
-    if graddev is None:
-        L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), stream))
-    else:
-        if on_dev:
-            raise ValueError("graddev fits take host arrays")
-        gd, _keep_gd = _graddev_struct(graddev, (nx, ny, nz), g.n_grads, bvecs_vox, frame_rotation,
-                                       graddev_variant)
-        L.check(L.lib().shorefit_fit_graddev(C.byref(g), C.byref(v), C.byref(gd), C.byref(spec), C.byref(out),
-                                             None))
+    if graddev is not None and on_dev:
+        raise ValueError("graddev fits take host arrays")
+    with _progress_hook(progress, progress_interval):
+        if graddev is None:
+            L.check(L.lib().shorefit_fit(C.byref(g), C.byref(v), C.byref(spec), C.byref(out), stream))
+        else:
+            gd, _keep_gd = _graddev_struct(graddev, (nx, ny, nz), g.n_grads, bvecs_vox, frame_rotation,
+                                           graddev_variant)
+            L.check(L.lib().shorefit_fit_graddev(C.byref(g), C.byref(v), C.byref(gd), C.byref(spec),
+                                                 C.byref(out), None))
     dist = tuple(out.alpha_distribution) if out.has_alpha_distribution else None
     result = {
         "coefficients": coefs,
@@ -267,6 +274,29 @@ def shore_fit(data, bvals, bvecs, mask, *, big_delta=None, small_delta=None, b0_
         if rss_l2 is not None:
             result["rss_l2"] = rss_l2
     return result
+
+
+@contextlib.contextmanager
+def _progress_hook(progress, interval):
+    """Registers ``progress`` with the library for the calls inside the block."""
+    if progress is None:
+        yield
+        return
+    if not callable(progress):
+        raise ValueError("progress must be callable(done, total)")
+    interval = float(interval)
+    if not interval > 0.0:
+        raise ValueError(f"progress_interval must be positive, got {interval}")
+
+    def relay(done, total, _user):
+        progress(int(done), int(total))
+
+    fn = L.PROGRESS_FN(relay)  # held here until the library lets go of it
+    L.check(L.lib().shorefit_set_progress(fn, None, interval))
+    try:
+        yield
+    finally:
+        L.lib().shorefit_set_progress(L.PROGRESS_FN(), None, 0.0)
 
 
 def _graddev_struct(graddev, shape, n_grads, bvecs_vox=None, frame_rotation=None, variant=-1):

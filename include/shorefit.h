@@ -140,6 +140,17 @@ void shorefit_spec_default(ShorefitSpec* spec);
 int shorefit_fit(const ShorefitGradients* grads, const ShorefitVolume* vol, const ShorefitSpec* spec,
                  ShorefitOutput* out, void* stream);
 
+/* Progress of whole-volume fits on the calling thread: the role of fit_shore's
+ * on_voxel callback behind cs-fit's heartbeat (src/fit.rs:69-73,
+ * src/bin/cs-fit.rs:286-287, src/progress.rs:17-80). The callback receives
+ * (voxels finished, masked voxels, user): once with 0 when the solve starts,
+ * every interval_seconds while it runs (counts the solve kernel publishes
+ * into host memory, polled by this thread), and once with the total after
+ * the fit. It is always called on the thread that called shorefit_fit, never
+ * from device code. fn = NULL switches reporting off. */
+typedef void (*shorefit_progress_fn)(int64_t voxels_done, int64_t voxels_total, void* user);
+int shorefit_set_progress(shorefit_progress_fn fn, void* user, double interval_seconds);
+
 /* ------------------------------------------------ gradient deviation ----
  * A gradient-deviation ("graddev") field gives, per voxel, a row-major 3x3 T
  * with g_eff = T^T g in the field's voxel axes (src/graddev.rs:1-24). Each
@@ -334,6 +345,12 @@ const char* shorefit_last_error(void);
 
 /* Library version string. */
 const char* shorefit_version(void);
+
+/* Build facts for provenance records, as a JSON object: version, git SHA of
+ * the sources (with "-dirty" for uncommitted changes), UTC build time and
+ * offload architecture. Replaces the compile-time VERSION / GIT_SHA /
+ * BUILD_TIMESTAMP the reference bakes in (src/lib.rs, build.rs:1-37). */
+const char* shorefit_build_info(void);
 
 /* Returns the calling thread's cached device blocks to the driver (after a
  * device synchronization). Entry points keep the blocks they release for

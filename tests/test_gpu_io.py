@@ -107,7 +107,13 @@ def test_fit_files_matches_shore_fit(tmp_path):
                        small_delta=g.small_delta, bvec_frame="world-ras", **kw)
     coef, meta = sio.read_coefficients(out)
     np.testing.assert_array_equal(coef, ref["coefficients"])
-    assert meta == json.loads(json.dumps(ref["sidecar"])) and meta == side
+    assert meta == json.loads(json.dumps(side))
+    # the fit's sidecar plus the provenance record (src/io/provenance.rs:34-105)
+    prov = meta.pop("provenance")
+    assert meta == json.loads(json.dumps(ref["sidecar"]))
+    assert set(prov) == {"tool", "version", "git_sha", "build_timestamp", "threads_used", "duration_secs"}
+    assert prov["tool"] == "cs-fit" and prov["threads_used"] == 1 and 0 < prov["duration_secs"] < 600
+    assert prov["version"] == "0.1.0" and prov["git_sha"]
     assert meta["bvec_frame"] == "world-ras" and meta["n_coefficients"] == 72
     assert nf.load(out).affine().tolist() == nf.load(dwi).affine().tolist()
     for key in ("r2", "rmse", "alpha", "bic"):
@@ -117,6 +123,12 @@ def test_fit_files_matches_shore_fit(tmp_path):
     np.testing.assert_array_equal(sp, sio.sparsity_map(ref["coefficients"], mask))
     with pytest.raises(FileExistsError):
         sio.fit_files(dwi, bval, bvec, out, mask=m, big_delta=g.big_delta, small_delta=g.small_delta, **kw)
+    # provenance "full" adds the command line, host and start time; "none" drops it
+    full = sio.fit_files(dwi, bval, bvec, tmp_path / "f.nii", mask=m, big_delta=g.big_delta,
+                         small_delta=g.small_delta, provenance="full", **kw)["provenance"]
+    assert {"argv", "hostname", "started_at_utc"} <= set(full)
+    assert "provenance" not in sio.fit_files(dwi, bval, bvec, tmp_path / "n.nii", mask=m, big_delta=g.big_delta,
+                                             small_delta=g.small_delta, provenance="none", **kw)
     # the whole path on the device
     out2 = tmp_path / "coef_dev.nii"
     sio.fit_files(dwi, bval, bvec, out2, mask=m, bvec_frame="world-ras", big_delta=g.big_delta,

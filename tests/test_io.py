@@ -11,6 +11,7 @@ The device layout conversion is covered in test_gpu_io.py.
 
 import json
 import struct
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -242,3 +243,89 @@ def test_sidecar_json_shape():
     assert sc["bvec_frame"] == "world-ras" and sc["dwi_shells"] == [1000.0, 2000.0]
     assert "provenance" not in sc
     json.dumps(sc)
+
+
+def test_ensure_nifti_extension(capsys):
+    # src/io/output.rs:461-486
+    assert sio.ensure_nifti_extension("/tmp/foo.nii.gz") == Path("/tmp/foo.nii.gz")
+    assert sio.ensure_nifti_extension("/tmp/foo.nii") == Path("/tmp/foo.nii")
+    assert sio.ensure_nifti_extension("/tmp/foo") == Path("/tmp/foo.nii.gz")
+    assert sio.ensure_nifti_extension("foo.tar.gz") == Path("foo.tar.gz.nii.gz")
+    assert "appending .nii.gz" in capsys.readouterr().err
+
+
+def test_precheck_writable(tmp_path):
+    # src/io/output.rs:488-524: a missing parent is created and the probe
+    # removed; an existing file is refused without overwrite and untouched
+    target = tmp_path / "nested" / "sub" / "out.nii.gz"
+    sio.precheck_writable(target, False)
+    assert target.parent.is_dir() and not any("precheck" in p.name for p in target.parent.iterdir())
+    target.write_bytes(b"existing")
+    with pytest.raises(FileExistsError, match="refusing to overwrite"):
+        sio.precheck_writable(target, False)
+    sio.precheck_writable(target, True)
+    assert target.read_bytes() == b"existing"
+
+
+def test_precheck_outputs_set_follows_the_fit(tmp_path):
+    out = tmp_path / "coef.nii.gz"
+    names = lambda **kw: sorted(p.name for p in sio.fit_output_paths(out, **kw))  # noqa: E731
+    assert names() == ["coef.json", "coef.nii.gz"]
+    assert names(diagnostics=True, regularization="l2") == sorted(
+        ["coef.nii.gz", "coef.json", "coef_r2.nii.gz", "coef_rmse.nii.gz", "coef_sparsity.nii.gz"])
+    assert "coef_rss_l2.nii.gz" in names(diagnostics=True, alpha_mode="l2-anchored")
+    assert "coef_rss_l2.nii.gz" not in names(diagnostics=True, alpha_mode="fixed")
+    assert {"coef_alpha.nii.gz", "coef_bic.nii.gz"} <= set(names(diagnostics=True, alpha_mode="fixed"))
+
+
+def test_fit_files_checks_outputs_before_reading(tmp_path):
+    # an existing diagnostic map stops the run before the inputs are read
+    # (the inputs here do not even exist), so no partial output set is left
+    out = tmp_path / "coef.nii.gz"
+    (tmp_path / "coef_bic.nii.gz").write_bytes(b"old")
+    with pytest.raises(FileExistsError, match="coef_bic"):
+        sio.fit_files(tmp_path / "missing.nii.gz", tmp_path / "b.bval", tmp_path / "b.bvec", out,
+                      alpha_mode="fixed", diagnostics=True)
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["coef_bic.nii.gz"]
+    with pytest.raises(ValueError, match="provenance"):
+        sio.ProvenanceRecorder("cs-fit", "verbose")
+
+
+# ---- progress heartbeat (src/progress.rs:98-143, tests :147-185) -------------
+def test_format_hms():
+    assert sio.format_hms(0) == "00:00:00"
+    assert sio.format_hms(59.9) == "00:00:59"
+    assert sio.format_hms(3600 + 2 * 60 + 3) == "01:02:03"
+    assert sio.format_hms(100 * 3600) == "100:00:00"
+
+
+def test_heartbeat_quiet_prints_nothing(capsys):
+    hb = sio.Heartbeat("t", 100, 1e-9, quiet=True)
+    for i in range(101):
+        hb.update(i, 100)
+    hb.finish()
+    assert hb.done == 100
+    assert capsys.readouterr().err == ""
+
+
+def test_heartbeat_lines_and_eta(capsys, monkeypatch):
+    clock = iter([0.0, 10.0, 10.5, 30.0, 40.0])
+    monkeypatch.setattr(sio.time, "perf_counter", lambda: next(clock))
+    hb = sio.Heartbeat("cs-fit", 1000, 5.0)  # t0 = 0
+    hb.update(250, 1000)   # t = 10: 250 in 10 s, 750 left -> eta 30 s
+    hb.update(300, 1000)   # t = 10.5: inside the interval, silent
+    hb.update(200, 1000)   # t = 30: counts never go backwards
+    hb.finish()            # t = 40
+    err = capsys.readouterr().err.splitlines()
+    assert err == ["[cs-fit] 250/1000 (25.0%) elapsed=00:00:10 eta=00:00:30",
+                   "[cs-fit] 300/1000 (30.0%) elapsed=00:00:30 eta=00:01:10",
+                   "[cs-fit] done 300/1000 (30.0%) elapsed=00:00:40"]
+
+
+def test_heartbeat_empty_mask_and_bad_interval(capsys):
+    hb = sio.Heartbeat("cs-fit", 0, 1e-9)
+    hb.update(0, 0)
+    hb.finish()
+    assert capsys.readouterr().err.splitlines() == ["[cs-fit] done 0/0 (100.0%) elapsed=00:00:00"]
+    with pytest.raises(ValueError):
+        sio.Heartbeat("cs-fit", 1, 0.0)

@@ -126,6 +126,22 @@ def test_refusals_are_value_errors():
         shore_fit(data, bvals, bvecs[:, :2], mask)
 
 
+def test_progress_registration_checks():
+    # no device call: registration only
+    lib = L.lib()
+    fn = L.PROGRESS_FN(lambda d, t, u: None)
+    assert lib.shorefit_set_progress(fn, None, 0.0) == L.E_INVALID
+    assert "interval" in lib.shorefit_last_error().decode()
+    assert lib.shorefit_set_progress(fn, None, 0.5) == 0
+    assert lib.shorefit_set_progress(L.PROGRESS_FN(), None, 0.0) == 0  # NULL switches it off
+    from cs_dmri_amd import shore_fit
+    data, bvals, bvecs, mask = _series([1000.0, 2000.0])
+    with pytest.raises(ValueError, match="callable"):
+        shore_fit(data, bvals, bvecs, mask, progress=3)
+    with pytest.raises(ValueError, match="progress_interval"):
+        shore_fit(data, bvals, bvecs, mask, progress=print, progress_interval=0)
+
+
 def test_nonneg_is_reported_unsupported():
     from cs_dmri_amd import shore_fit
     data, bvals, bvecs, mask = _series([1000.0, 2000.0])
