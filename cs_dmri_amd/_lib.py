@@ -8,11 +8,14 @@ built ``libshorefit.so`` is missing, and every call raises ``ValueError`` /
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libshorefit.so"
+# SHOREFIT_LIB selects another build of the library (the host-sanitizer build)
+LIB_PATH = (Path(os.environ["SHOREFIT_LIB"]) if os.environ.get("SHOREFIT_LIB")
+            else Path(__file__).resolve().parent / "libshorefit.so")
 
 OK, E_INVALID, E_FIT, E_NOT_PD, E_UNSUPPORTED, E_DEVICE = range(6)
 REG = {"l1": 1, "l2": 2, "nonneg": 3}

@@ -92,5 +92,55 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+# Host-code sanitizers (AddressSanitizer + UndefinedBehaviorSanitizer) for
+# the C-ABI layer: every flag goes to the host pass only, the device code is
+# built as usual.
+SAN_HOST_FLAGS = ["-O1", "-g",
+                  "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer"]
+SAN_LIB = PKG / "_san" / "libshorefit.so"
+
+
+def asan_runtime() -> Path:
+    """The clang AddressSanitizer runtime the sanitized library links against
+    (preload it into the host process)."""
+    cc = Path(hipcc()).resolve()
+    for root in (cc.parent.parent / "llvm" / "lib" / "clang", cc.parent.parent / "lib" / "clang",
+                 Path("/opt/rocm/llvm/lib/clang")):
+        hits = sorted(root.glob("*/lib/linux/libclang_rt.asan-x86_64.so"))
+        if hits:
+            return hits[-1]
+    raise RuntimeError("libclang_rt.asan-x86_64.so not found next to hipcc")
+
+
+def build_host_sanitized(verbose: bool = False) -> Path:
+    """``_san/libshorefit.so``: api.cpp (argument checks, gradient table,
+    shells, caches, progress) built with host sanitizers and linked with the
+    kernel objects of the regular build (run ``build()`` first)."""
+    cc = hipcc()
+    objdir = PKG / "_build"
+    kernels = [objdir / (Path(src).stem + ".o") for src in SOURCES if src != "api.cpp"]
+    missing = [str(o) for o in kernels if not o.exists()]
+    if missing:
+        raise RuntimeError(f"kernel objects missing (run the regular build first): {missing}")
+    out = SAN_LIB.parent
+    out.mkdir(exist_ok=True)
+    api = out / "api.o"
+    fl = [f for f in flags(build_stamp()) if f != "-O3"]
+    cmd = [cc, *fl, *SAN_HOST_FLAGS, "-x", "hip", "-c", str(CSRC / "api.cpp"), "-o", str(api)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, capture_output=not verbose)
+    tmp = SAN_LIB.with_suffix(".so.tmp")
+    link = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), str(api), *map(str, kernels),
+            "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-shared-libsan"]
+    subprocess.run(link, check=True, capture_output=not verbose)
+    os.replace(tmp, SAN_LIB)
+    return SAN_LIB
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--sanitize-host" in sys.argv:
+        print(build_host_sanitized(verbose="-v" in sys.argv))
+    else:
+        print(build(verbose="-v" in sys.argv))

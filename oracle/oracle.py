@@ -14,7 +14,8 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-_LIB = _HERE / "liboracle_shore.so"
+# ORACLE_LIB selects another build of the same source (the sanitizer build)
+_LIB = Path(os.environ["ORACLE_LIB"]) if os.environ.get("ORACLE_LIB") else _HERE / "liboracle_shore.so"
 
 _d = C.c_double
 _i = C.c_int
