@@ -13,6 +13,8 @@ Tolerances
   distribution: median <= 1e-5, 99th percentile <= 1e-3, and the per-voxel
   iteration counts agree within 10% for 95% of voxels.
 * Tikhonov (FP64 factorisation, FP32 application): 1e-4 relative per voxel.
+* sklearn fixtures and the KKT problems: the reference's own bounds
+  (tests/optimization_sklearn.rs:159-202, tests/optimization_kkt.rs:39-156).
 """
 
 import numpy as np
@@ -22,7 +24,8 @@ pytestmark = pytest.mark.gpu
 
 import cs_dmri_amd as sf  # noqa: E402
 from cs_dmri_amd import workloads as W  # noqa: E402
-from tests.fixtures import kkt_lasso, kkt_nonneg_lasso, lasso_fixtures, lasso_objective, shell_gtab  # noqa: E402
+from tests.fixtures import (kkt_lasso, kkt_nonneg_lasso, lasso_fixtures, lasso_objective, make_design,  # noqa: E402
+                            make_ill_conditioned, make_noisy_signal, make_planted_beta, shell_gtab)
 
 
 def _rel(c, ref):
@@ -84,11 +87,18 @@ def test_fista_stopping_rule_config1(orc, cfg1):
     np.testing.assert_allclose(out["residual_l2"][wl.mask], ref["residual_l2"][wl.mask], rtol=1e-3)
     assert (out["regularization_kind"][wl.mask] == 1).all()
     assert np.all(out["alpha"][wl.mask] == np.float32(1e-3))
-    # every voxel whose FP64 fit converged is a near-KKT point in FP64 arithmetic
+    # The relative-change rule stops short of the exact minimiser in either
+    # arithmetic, so the device's KKT residual is held to the FP64 oracle's on
+    # the same voxel (plus the FP32 floor), not to an absolute bound.
     S = wl.data[wl.mask].astype(np.float64)
-    for v in range(0, len(S), 97):
-        act, ina = kkt_lasso(A, S[v], c[v], 1e-3, zero_tol=0.0)
-        assert max(act, ina) < 5e-3 * np.abs(A.T @ S[v]).max() / len(S[v])
+    worst = 0.0
+    for v in range(0, len(S), 31):
+        scale = np.abs(A.T @ S[v]).max() / len(S[v])
+        g = max(kkt_lasso(A, S[v], c[v], 1e-3))
+        r = max(kkt_lasso(A, S[v], cr[v], 1e-3))
+        worst = max(worst, (g - r) / scale)
+        assert g <= 2.0 * r + 1e-6 * scale, (v, g, r, scale)
+    print(f"worst (device - oracle) KKT residual / |A^T s|_inf/N: {worst:.3e}")
 
 
 FIXTURES = lasso_fixtures()
@@ -96,15 +106,72 @@ FIXTURES = lasso_fixtures()
 
 @pytest.mark.parametrize("fx", FIXTURES, ids=[f["name"] for f in FIXTURES])
 def test_fista_sklearn_fixtures(fx):
-    coef, d = sf.fista_batch(fx["A"], fx["y"][None], fx["alpha"], max_iter=20_000, tol=1e-9,
+    """The reference's settings and bounds (tests/optimization_sklearn.rs:159-202):
+    max_iter 200000, tol 1e-12, coefficients 1e-3, objective 1e-6, KKT 1e-5.
+    The FP32 device solve lands at objective gaps of 1e-15 and KKT residuals
+    of at most 1.1e-8 on these fixtures."""
+    coef, d = sf.fista_batch(fx["A"], fx["y"][None], fx["alpha"], max_iter=200_000, tol=1e-12,
                              non_negative=fx["positive"])
     c = coef[0].astype(np.float64)
     rel = np.linalg.norm(c - fx["beta"]) / max(np.linalg.norm(fx["beta"]), 1.0)
     assert rel < 1e-3
     obj = lasso_objective(fx["A"], fx["y"], c, fx["alpha"])
-    assert abs(obj - fx["objective"]) / max(abs(fx["objective"]), 1.0) < 1e-5
-    kkt = (kkt_nonneg_lasso if fx["positive"] else kkt_lasso)(fx["A"], fx["y"], c, fx["alpha"], zero_tol=0.0)
-    assert max(kkt) < 1e-4
+    assert abs(obj - fx["objective"]) / max(abs(fx["objective"]), 1.0) < 1e-6
+    active, inactive = (kkt_nonneg_lasso if fx["positive"] else kkt_lasso)(fx["A"], fx["y"], c, fx["alpha"])
+    assert active < 1e-5 and inactive < 1e-5, (active, inactive)
+
+
+def _kkt_case(kind):
+    """The problems of tests/optimization_kkt.rs:39-156 (numpy's generator in
+    place of ChaCha8, so the draws differ; shapes, kappa, sparsity, noise and
+    alpha as there)."""
+    if kind == "well":
+        A, beta, seed, nn = make_design(200, 50, 0xA1), make_planted_beta(50, 8, True, 0xA2), 0xA3, False
+    elif kind == "ill":
+        A, beta, seed, nn = make_ill_conditioned(150, 40, 1e4, 0xB1), make_planted_beta(40, 6, True, 0xB2), 0xB3, False
+    elif kind == "under":
+        A, beta, seed, nn = make_design(30, 80, 0xC1), make_planted_beta(80, 5, True, 0xC2), 0xC3, False
+    elif kind == "over":
+        A, beta, seed, nn = make_design(400, 20, 0xD1), make_planted_beta(20, 4, True, 0xD2), 0xD3, False
+    elif kind == "nonneg":
+        A, beta, seed, nn = make_design(180, 45, 0xF1), make_planted_beta(45, 7, False, 0xF2), 0xF3, True
+    else:  # nonneg fit of a mixed-sign truth: the constraint binds
+        A, beta, seed, nn = make_design(200, 50, 0x11), make_planted_beta(50, 8, True, 0x12), 0x13, True
+    return A, make_noisy_signal(A, beta, 0.05, seed), nn
+
+
+# (case, KKT bound) as in tests/optimization_kkt.rs:39-156
+KKT_CASES = [("well", 1e-6), ("ill", 1e-3), ("under", 1e-6), ("over", 1e-6), ("nonneg", 1e-6),
+             ("nonneg-binding", 1e-5)]
+
+
+@pytest.mark.parametrize("kind,bound", KKT_CASES, ids=[k for k, _ in KKT_CASES])
+@pytest.mark.parametrize("alpha_rel", [None, 0.1], ids=["alpha0.05", "alpha0.1max"])
+def test_fista_kkt_cases(kind, bound, alpha_rel):
+    """KKT residuals of the device solve, at the reference's alpha = 0.05 and,
+    because with N(0, 1/m) designs 0.05 is above alpha_max for most of these
+    draws (a zero solution), also at 0.1 * alpha_max, where the solution has
+    an active set. Non-negative fits never go below zero."""
+    A, y, nn = _kkt_case(kind)
+    amax = np.abs(A.T @ y).max() / A.shape[0]
+    alpha = 0.05 if alpha_rel is None else alpha_rel * amax
+    coef, d = sf.fista_batch(A, y[None], alpha, max_iter=100_000, tol=1e-10, non_negative=nn)
+    c = coef[0].astype(np.float64)
+    if alpha_rel is not None:
+        assert np.count_nonzero(c) > 0
+    if nn:
+        assert c.min() >= 0.0
+    active, inactive = (kkt_nonneg_lasso if nn else kkt_lasso)(A, y, c, alpha)
+    assert active < bound and inactive < bound, (active, inactive, int(d["iterations"][0]))
+
+
+def test_fista_at_alpha_max_is_exact_zero():
+    """tests/optimization_kkt.rs:106-122: at alpha = alpha_max the solution is 0."""
+    A = make_design(120, 35, 0xE1)
+    y = make_noisy_signal(A, make_planted_beta(35, 5, True, 0xE2), 0.05, 0xE3)
+    amax = np.abs(A.T @ y).max() / A.shape[0]
+    coef, _ = sf.fista_batch(A, y[None], amax, max_iter=100_000, tol=1e-10)
+    assert np.abs(coef).max() < 1e-12
 
 
 def _small_volume():

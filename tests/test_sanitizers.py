@@ -63,6 +63,11 @@ def test_sanitizer_flags_catch_planted_bugs(tmp_path, kind):
     assert kind in r.stderr
 
 
+def _preload(rt):
+    # the sanitizer runtime goes first; whatever the environment preloads stays
+    return ":".join(p for p in (str(rt), os.environ.get("LD_PRELOAD", "")) if p)
+
+
 def _run_child(code, env, timeout=600):
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=timeout,
                        env={**os.environ, **ENV, **env})
@@ -79,7 +84,7 @@ def test_oracle_suite_under_sanitizers(san_oracle):
             "oracle.load()\n"
             f"assert {str(san_oracle)!r} in open('/proc/self/maps').read(), 'sanitized oracle not mapped'\n"
             f"sys.exit(pytest.main({tests!r} + ['-q', '-x', '-p', 'no:cacheprovider', '-m', 'not gpu']))\n")
-    out = _run_child(code, {"ORACLE_LIB": str(san_oracle), "LD_PRELOAD": _gcc_lib("libasan.so")})
+    out = _run_child(code, {"ORACLE_LIB": str(san_oracle), "LD_PRELOAD": _preload(_gcc_lib("libasan.so"))})
     assert " passed" in out
 
 
@@ -92,5 +97,5 @@ def test_abi_layer_under_host_sanitizers():
             f"assert {str(lib)!r} in open('/proc/self/maps').read(), 'sanitized library not mapped'\n"
             "sys.exit(pytest.main(['tests/test_lib_abi.py', '-q', '-x', '-p', 'no:cacheprovider', "
             "'-m', 'not gpu']))\n")
-    out = _run_child(code, {"SHOREFIT_LIB": str(lib), "LD_PRELOAD": str(B.asan_runtime())})
+    out = _run_child(code, {"SHOREFIT_LIB": str(lib), "LD_PRELOAD": _preload(B.asan_runtime())})
     assert " passed" in out
