@@ -211,7 +211,12 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
 // Workgroup = 8 waves x 64 lanes x VPL voxels per lane (signal chunks staged
 // in LDS); wave w owns the contiguous coefficient range [w KQ, (w+1) KQ) of
 // the zero-padded design Ap (N x Kp, FP64), which it reads with wave-uniform
-// (scalar) loads. Each loaded element feeds VPL FMAs per lane.
+// (scalar) loads. Each loaded element feeds VPL FMAs per lane. The next
+// signal chunk is loaded into registers (PER independent loads per thread,
+// each thread keeping its voxel and sample slot from chunk to chunk) while
+// the current chunk is summed, so the global-load latency is paid once per
+// workgroup instead of once per staged element (0.69 -> 0.55 ms on 250,176
+// voxels at N = 258, Kp = 160).
 template <int KQ, int VPL>  // KQ = Kp / 8 coefficients per wave
 __global__ void __launch_bounds__(512) rhs_kernel(const float* __restrict__ data, const int64_t* __restrict__ vox,
                                                   int64_t n_vox, int64_t sig_stride, int N, int Kp,
@@ -220,10 +225,30 @@ __global__ void __launch_bounds__(512) rhs_kernel(const float* __restrict__ data
     static_assert(KQ % 2 == 0, "coefficients are stored in pairs");
     constexpr int CH = 32;        // samples per LDS chunk
     constexpr int NV = 64 * VPL;  // voxels per workgroup
+    constexpr int NT = 512;
+    constexpr int PER = (NV * CH + NT - 1) / NT;  // staged elements per thread and chunk
     __shared__ float sm[NV][CH + 1];
-    const int lane = threadIdx.x & 63;
-    const int k0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * KQ;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int k0 = __builtin_amdgcn_readfirstlane(tid >> 6) * KQ;
     const int64_t v0 = int64_t(blockIdx.x) * NV;
+    // staging slot i of this thread: element e = tid + i NT of every chunk,
+    // i.e. voxel e / CH and sample offset e % CH
+    const float* src[PER];
+    int snn[PER], svj[PER];
+    bool sok[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * NT;
+        svj[i] = e / CH;
+        snn[i] = e % CH;
+        const int64_t v = v0 + svj[i];
+        sok[i] = e < NV * CH && v < n_vox;
+        src[i] = sok[i] ? data + (vox ? vox[v] : v) * sig_stride + snn[i] : data;
+    }
+    float pre[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) pre[i] = (sok[i] && snn[i] < N) ? src[i][0] : 0.f;
     double acc[VPL][KQ];
     double sum[VPL];
 #pragma unroll
@@ -234,14 +259,15 @@ __global__ void __launch_bounds__(512) rhs_kernel(const float* __restrict__ data
     }
     for (int n0 = 0; n0 < N; n0 += CH) {
         __syncthreads();
-        for (int t = threadIdx.x; t < NV * CH; t += 512) {
-            const int vj = t / CH, nn = t % CH;
-            const int64_t v = v0 + vj;
-            float x = 0.f;
-            if (v < n_vox && n0 + nn < N) x = data[(vox ? vox[v] : v) * sig_stride + n0 + nn];
-            sm[vj][nn] = x;
-        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+            if (tid + i * NT < NV * CH) sm[svj[i]][snn[i]] = pre[i];
         __syncthreads();
+        const int n1 = n0 + CH;
+        if (n1 < N) {  // the next chunk, in flight while this one is summed
+#pragma unroll
+            for (int i = 0; i < PER; ++i) pre[i] = (sok[i] && n1 + snn[i] < N) ? src[i][n1] : 0.f;
+        }
         const int nmax = min(CH, N - n0);
         for (int nn = 0; nn < nmax; ++nn) {
             double sv[VPL];
@@ -590,9 +616,300 @@ __global__ void __launch_bounds__(256) tikhonov_multi_kernel(const FistaArgs a, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Apply kernels with the fixed operand staged in LDS.
+//
+// The operand sequence is cut into chunks of MB 16x16x4 fragments: chunk g <
+// nc1 holds the fragments (mb, g) of ATfrag (the c = P s pass: one 16-sample
+// chunk of P, every coefficient block; chunks from nc then hold zeros), chunk
+// nc1 + nb the fragments (nb, kb) of Afrag (the residual pass: one 16-row
+// block of A, every coefficient block). A workgroup stages each chunk once in
+// LDS for all its waves, double buffered, and requests each chunk from L2 two
+// chunks before it is stored (two register sets), so the load latency is
+// covered by two chunks of products instead of every wave fetching every
+// fragment from L2 with the latency of each load exposed.
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima), gfx9 encoding.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+template <int MB, int NT>
+struct ChunkStage {
+    static constexpr int NF = MB * 64;  // floatx4 per chunk
+    static constexpr int PER = (NF + NT - 1) / NT;
+    floatx4 pre[2][PER];
+    // Branch-free: every lane issues its PER loads from in-bounds addresses
+    // (indices clamped, the zero chunk read from chunk 0 and zeroed after),
+    // so no exec-masked branch sits between a load and the products that
+    // follow it; such branches made the compiler wait for every outstanding
+    // load before the next chunk's products.
+    template <int SET>
+    __device__ __forceinline__ void load(const FistaArgs& a, int g, int nc1, int nc, int tid) {
+        const bool second = g >= nc1;
+        const bool zero = !second && g >= nc;
+        const floatx4* base = second ? a.Afrag + size_t(g - nc1) * MB * 64 : a.ATfrag + size_t(zero ? 0 : g) * 64;
+        const size_t smb = second ? 64 : size_t(a.NC) * 64;  // stride between coefficient blocks
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = min(tid + i * NT, NF - 1);
+            const floatx4 f = base[size_t(e >> 6) * smb + (e & 63)];
+            pre[SET][i] = zero ? floatx4{0.f, 0.f, 0.f, 0.f} : f;
+        }
+    }
+    template <int SET>
+    __device__ __forceinline__ void store(floatx4* buf, int tid) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + i * NT;
+            if (e < NF) buf[e] = pre[SET][i];
+        }
+    }
+};
+
+// Tikhonov L2 (src/solver/tikhonov.rs:59-76) with the signals read once:
+// c = P s accumulates over the 16-sample chunks of each wave's 16 voxels,
+// and the chunks stay in registers (4 samples per lane and chunk, in the
+// layout the residual pass needs: rows 16 nb + 4h + r of voxel j on lane
+// 16h + j), so |s - A c|^2 and the sum of squares about the mean use them
+// without a second pass over the signals. NCM bounds the chunk count
+// (N <= 16 NCM); W waves of 16 voxels per workgroup. The c = P s pass runs
+// over an even number of chunks (a zero chunk after P's last when NC is
+// odd), so that chunk parities, and with them the LDS buffer and register
+// set of every step, are compile-time constants.
+template <int MB, int NCM, int W>
+__global__ void __launch_bounds__(64 * W) tikhonov_lds_kernel(const FistaArgs a, float* rss_l2) {
+    static_assert(NCM % 2 == 0, "the first pass runs over pairs of chunks");
+    __shared__ floatx4 fb[2][MB * 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    const int64_t v = (int64_t(blockIdx.x) * W + wave) * 16 + j;
+    const bool valid = v < a.n_vox;
+    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
+    const float* sig = a.data + out_idx * a.sig_stride;
+    const int NC = a.NC;             // <= NCM, checked by the host
+    const int NC1 = (NC + 1) & ~1;   // first-pass chunks (even)
+    const int total = NC1 + NC;
+    ChunkStage<MB, 64 * W> st;
+    st.template load<0>(a, 0, NC1, NC, tid);
+    st.template load<1>(a, 1, NC1, NC, tid);  // total >= 3
+    // every signal load of the tile is issued here, ahead of the products
+    // (loops fully unrolled, so sreg stays in registers)
+    float sreg[NCM][4];
+    // Unguarded: chunks before the last are inside the signal (NC =
+    // ceil(N / 16)) and load from one base with immediate offsets; in the
+    // last chunk, rows past N read sample N - 1; they meet zero columns of
+    // P in the first pass and are left out of the sums. A per-lane guard (a
+    // select or an exec-masked load) would make the compiler wait for each
+    // load at once, serializing all of them.
+    const float* sb = sig + 4 * h;
+#pragma unroll
+    for (int ch = 0; ch < NCM; ++ch) {
+        if (ch + 1 < NC) {  // uniform
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sreg[ch][r] = sb[16 * ch + r];
+        } else if (ch + 1 == NC) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sreg[ch][r] = sig[min(16 * ch + 4 * h + r, a.N - 1)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sreg[ch][r] = 0.f;
+        }
+    }
+    st.template store<0>(fb[0], tid);
+    __syncthreads();
+    // one step of the chunk pipeline: request chunk g + 2 into the register
+    // set chunk g came from, multiply chunk g, store chunk g + 1
+    auto step_begin = [&](int g, auto par) {
+        constexpr int P = decltype(par)::value;
+        if (g + 2 < total) st.template load<P>(a, g + 2, NC1, NC, tid);
+    };
+    auto step_end = [&](int g, auto par) {
+        constexpr int P = decltype(par)::value;
+        if (g + 1 < total) st.template store<1 - P>(fb[1 - P], tid);
+        __syncthreads();
+    };
+    floatx4 c[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) c[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float sum = 0.f;
+#pragma unroll
+    for (int ch = 0; ch < NCM; ++ch) {
+        if (ch < NC1) {
+            if (ch & 1) step_begin(ch, std::integral_constant<int, 1>{});
+            else step_begin(ch, std::integral_constant<int, 0>{});
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sum += (16 * ch + 4 * h + r < a.N) ? sreg[ch][r] : 0.f;
+            const floatx4* f = fb[ch & 1];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const floatx4 af = f[mb * 64 + lane];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) c[mb] = mfma4(af[r], sreg[ch][r], c[mb]);
+            }
+            if (ch & 1) step_end(ch, std::integral_constant<int, 1>{});
+            else step_end(ch, std::integral_constant<int, 0>{});
+        }
+    }
+    const float mean = lane_sum4(sum) / float(a.N);
+    float rss = 0.f, sst = 0.f;
+#pragma unroll
+    for (int nb = 0; nb < NCM; ++nb) {
+        if (nb < NC) {
+            const int g = NC1 + nb;  // same parity as nb
+            if (nb & 1) step_begin(g, std::integral_constant<int, 1>{});
+            else step_begin(g, std::integral_constant<int, 0>{});
+            const floatx4* f = fb[nb & 1];
+            // four independent accumulation chains (one per k-slot r)
+            floatx4 ac[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ac[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < MB; ++kb) {
+                const floatx4 af = f[kb * 64 + lane];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ac[r] = mfma4(af[r], c[kb][r], ac[r]);
+            }
+            const floatx4 acc = (ac[0] + ac[1]) + (ac[2] + ac[3]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int n = 16 * nb + 4 * h + r;
+                if (valid && n < a.N) {
+                    const float e = sreg[nb][r] - acc[r];
+                    rss = fmaf(e, e, rss);
+                    const float dm = sreg[nb][r] - mean;
+                    sst = fmaf(dm, dm, sst);
+                }
+            }
+            if (nb & 1) step_end(g, std::integral_constant<int, 1>{});
+            else step_end(g, std::integral_constant<int, 0>{});
+        }
+    }
+    rss = lane_sum4(rss);
+    sst = lane_sum4(sst);
+    if (!valid) return;
+    if (a.coef)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * mb + 4 * h + r;
+                if (k < a.K) a.coef[out_idx * a.K + k] = c[mb][r];
+            }
+    if (h != 0) return;
+    const double ss_res = double(rss);
+    if (a.resid) a.resid[out_idx] = float(sqrt(ss_res));
+    if (a.r2) a.r2[out_idx] = float(sst > 0.f ? 1.0 - ss_res / double(sst) : 0.0);
+    if (a.iters) a.iters[out_idx] = 1;
+    if (a.converged) a.converged[out_idx] = 1;
+    if (a.kind) a.kind[out_idx] = 2;
+    if (rss_l2) rss_l2[out_idx] = float(ss_res);
+}
+
+// Epilogue of the persistent solvers (residual, R^2, BIC, scatter of the
+// iterates they wrote in work order) with A staged in LDS by chunks of 16
+// rows; each wave takes 16 voxels and reads their signals once, each
+// chunk's signals requested one chunk ahead. The chunk loop is unrolled by
+// two so that buffers and register sets are compile-time constants.
+template <int MB, int W>
+__global__ void __launch_bounds__(64 * W) epilogue_lds_kernel(const FistaArgs a, const float* X, const uint32_t* its,
+                                                              const uint8_t* status, const float* alpha_v) {
+    __shared__ floatx4 fb[2][MB * 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    const int64_t v = (int64_t(blockIdx.x) * W + wave) * 16 + j;
+    const bool valid = v < a.n_vox;
+    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
+    const float* sig = a.data + out_idx * a.sig_stride;
+    const int NC = a.NC;
+    ChunkStage<MB, 64 * W> st;
+    st.template load<0>(a, 0, 0, 0, tid);
+    if (NC > 1) st.template load<1>(a, 1, 0, 0, tid);
+    floatx4 x[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+        x[mb] = valid ? *reinterpret_cast<const floatx4*>(X + v * a.Kp + 16 * mb + 4 * h)
+                      : floatx4{0.f, 0.f, 0.f, 0.f};
+    const float mean = valid ? a.mean[v] : 0.f;
+    float rss = 0.f, sst = 0.f;
+    // Rows past N (and chunks past the last) read sample N - 1 and are left
+    // out of the sums below, so the loads need no guard: an exec-masked load
+    // or a select right after it would make the compiler wait for the load
+    // at once instead of where the value is used.
+    auto sload = [&](int nb, float (&s4)[4]) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s4[r] = sig[min(16 * nb + 4 * h + r, a.N - 1)];
+    };
+    auto chunk = [&](int nb, const float (&s4)[4], auto par) {
+        constexpr int P = decltype(par)::value;
+        if (nb + 2 < NC) st.template load<P>(a, nb + 2, 0, 0, tid);
+        const floatx4* f = fb[P];
+        floatx4 ac[4];  // four independent accumulation chains (one per k-slot r)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ac[r] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < MB; ++kb) {
+            const floatx4 af = f[kb * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ac[r] = mfma4(af[r], x[kb][r], ac[r]);
+        }
+        const floatx4 acc = (ac[0] + ac[1]) + (ac[2] + ac[3]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = 16 * nb + 4 * h + r;
+            if (valid && n < a.N) {
+                const float e = s4[r] - acc[r];
+                rss = fmaf(e, e, rss);
+                const float dm = s4[r] - mean;
+                sst = fmaf(dm, dm, sst);
+            }
+        }
+        if (nb + 1 < NC) st.template store<1 - P>(fb[1 - P], tid);
+        __syncthreads();
+    };
+    // x, the mean and the first chunk are complete before the loop (one
+    // wait here; left to itself the compiler keeps a full wait for the
+    // loads of x inside every step)
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    float sa[4], sb[4];
+    sload(0, sa);
+    st.template store<0>(fb[0], tid);
+    __syncthreads();
+    for (int nb = 0; nb < NC; nb += 2) {
+        sload(nb + 1, sb);
+        chunk(nb, sa, std::integral_constant<int, 0>{});
+        if (nb + 1 < NC) {
+            sload(nb + 2, sa);
+            chunk(nb + 1, sb, std::integral_constant<int, 1>{});
+        }
+    }
+    rss = lane_sum4(rss);
+    sst = lane_sum4(sst);
+    if (!valid) return;
+    const uint8_t st_v = status[v];
+    write_voxel<MB>(a, out_idx, h, x, rss, sst, its[v], st_v & 1, double(alpha_v[v]), (st_v & 2) != 0);
+}
+
+// Chunks of the LDS Tikhonov kernel's signal registers: N <= 16 * 18 = 288.
+constexpr int kTikLdsChunks = 18;
+
+template <int MB>
+void launch_epilogue(const FistaArgs& args, const float* X, const uint32_t* its, const uint8_t* status,
+                     const float* alpha_v, hipStream_t s) {
+    constexpr int W = 4;
+    const int64_t tiles = (args.n_vox + 15) / 16;
+    hipLaunchKernelGGL((epilogue_lds_kernel<MB, W>), dim3(unsigned((tiles + W - 1) / W)), dim3(64 * W), 0, s, args, X,
+                       its, status, alpha_v);
+    SF_HIP(hipGetLastError());
+}
+
 template <int MB>
 void launch_tik(const FistaArgs& args, int64_t n_vox, float* rss_l2, hipStream_t s) {
     const int64_t tiles = (n_vox + 15) / 16;
+    if (args.NC <= kTikLdsChunks) {
+        constexpr int W = 4;
+        hipLaunchKernelGGL((tikhonov_lds_kernel<MB, kTikLdsChunks, W>), dim3(unsigned((tiles + W - 1) / W)),
+                           dim3(64 * W), 0, s, args, rss_l2);
+        SF_HIP(hipGetLastError());
+        return;
+    }
     if constexpr (MB >= 8 && MB <= 10) {
         constexpr int T = 2;
         hipLaunchKernelGGL((tikhonov_multi_kernel<MB, T>), dim3(unsigned((tiles + 4 * T - 1) / (4 * T))), dim3(256),
@@ -622,7 +939,7 @@ void launch_rhs(const FistaLaunch& L, hipStream_t s) {
 // the coefficients and iteration counts are the same bit for bit.
 // Each voxel's outputs are written as it stops: the coefficients and
 // diagnostics directly, or (when residual maps are wanted) the iterate and
-// its status in work order for fista_epilogue_kernel, as the mixed kernel
+// its status in work order for epilogue_lds_kernel, as the mixed kernel
 // does.
 template <int MB, int WAVES, bool M_IN_LDS, bool PATH>
 __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const FistaArgs a, unsigned int* counter,
@@ -655,7 +972,7 @@ __global__ void __launch_bounds__(64 * WAVES) fista_f32_persist_kernel(const Fis
 
     // outputs of the slot's voxel (src/solver/fista.rs:113-143 diagnostics):
     // with X, the iterate, iterations, status (bit 0 converged, bit 1 zero
-    // voxel) and alpha in work order, for fista_epilogue_kernel; without,
+    // voxel) and alpha in work order, for epilogue_lds_kernel; without,
     // the coefficients and diagnostics directly
     auto finish = [&](bool conv) {
         if (X) {
@@ -1418,29 +1735,6 @@ __global__ void __launch_bounds__(64 * WAVES) fista_mixed_kernel(const MixedArgs
     }
 }
 
-// Batched epilogue of the mixed form: residual, R^2, BIC, scatter.
-template <int MB>
-__global__ void __launch_bounds__(256) fista_epilogue_kernel(const FistaArgs a, const float* X, const uint32_t* its,
-                                                             const uint8_t* status, const float* alpha_v) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int h = lane >> 4, j = lane & 15;
-    const int64_t v = (int64_t(blockIdx.x) * 4 + wave) * 16 + j;
-    const bool valid = v < a.n_vox;
-    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
-    const float* sig = a.data + out_idx * a.sig_stride;
-    floatx4 x[MB];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-        x[mb] = valid ? *reinterpret_cast<const floatx4*>(X + v * a.Kp + 16 * mb + 4 * h)
-                      : floatx4{0.f, 0.f, 0.f, 0.f};
-    const float mean = valid ? a.mean[v] : 0.f;
-    float rss, sst;
-    residual<MB>(a, sig, valid, h, lane, x, mean, rss, sst);
-    if (!valid) return;
-    const uint8_t st = status[v];
-    write_voxel<MB>(a, out_idx, h, x, rss, sst, its[v], st & 1, double(alpha_v[v]), (st & 2) != 0);
-}
-
 // Selection pass of the alpha-path strategies (src/solver/alpha.rs:107-165,
 // 244-263, 285-316): per voxel, the residual sum of squares and non-zero count
 // of every path point, then the BIC argmin (ties to the larger alpha) or the
@@ -1562,10 +1856,7 @@ void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t
         return;
     }
     if (!L.Xbuf) return;
-    const int64_t tiles = (args.n_vox + 15) / 16;
-    hipLaunchKernelGGL(fista_epilogue_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, args, L.Xbuf,
-                       L.itsbuf, L.statusbuf, L.alphabuf);
-    SF_HIP(hipGetLastError());
+    launch_epilogue<MB>(args, L.Xbuf, L.itsbuf, L.statusbuf, L.alphabuf, s);
 }
 
 template <int MB, bool BF16>
@@ -1638,10 +1929,7 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
         launch_path_select<MB>(ea, s);
         return;
     }
-    const int64_t tiles = (L.n_vox + 15) / 16;
-    hipLaunchKernelGGL(fista_epilogue_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, ea, L.Xbuf,
-                       L.itsbuf, L.statusbuf, L.alphabuf);
-    SF_HIP(hipGetLastError());
+    launch_epilogue<MB>(ea, L.Xbuf, L.itsbuf, L.statusbuf, L.alphabuf, s);
 }
 
 }  // namespace
