@@ -295,10 +295,10 @@ def tikhonov_bench(args):
     Tikhonov L2 solver, lambda_N = lambda_L = 1e-2 (src/solver/tikhonov.rs:26-76).
     A step is one whole ``shorefit_fit``: design, G, Cholesky of G + Lambda,
     P = (G + Lambda)^-1 A^T (FP64, on the device), then c = P s and the
-    residual per voxel on FP32 MFMA (``tikhonov_kernel``). The roofline is that
-    kernel's: 4 N K algorithmic flops per voxel (P s and A c) against the FP32
-    MFMA peak; its compulsory bytes, 4 (N + K + 1) per voxel, are reported
-    beside it."""
+    residual per voxel on FP32 MFMA (``tikhonov_lds_kernel``, signals read
+    once). The roofline is that kernel's: 4 N K algorithmic flops per voxel
+    (P s and A c) against the FP32 MFMA peak; its compulsory bytes,
+    4 (N + K + 1) per voxel, are reported beside it."""
     import torch
 
     from cs_dmri_amd import _lib as L
@@ -351,7 +351,7 @@ def tikhonov_bench(args):
         "roofline": {"bound": "mfma", "achieved": round(flops / kern_s / 1e12, 3), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": round(flops / kern_s / 1e12 / PEAK["fp32"], 4),
                      "traffic": traffic_bytes("tikhonov"),
-                     "kernel": "tikhonov_multi_kernel<10,2>", "kernel_ms": round(kern_s * 1e3, 3),
+                     "kernel": "tikhonov_lds_kernel<10,18,4>", "kernel_ms": round(kern_s * 1e3, 3),
                      "flops_per_launch": flops, "algorithmic_bytes": nbytes,
                      "GB_s": round(nbytes / kern_s / 1e9, 1)},
     }
