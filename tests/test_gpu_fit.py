@@ -307,3 +307,26 @@ def test_shore_model_front_end(tmp_path):
     assert back.frame == "world" and back.sidecar == fw.sidecar and back.iterations is None
     np.testing.assert_allclose(back.affine, aff)
     np.testing.assert_array_equal(back.odf_sh(), fw.odf_sh())
+
+
+def test_tortoise_deltas_through_the_library(orc):
+    """Deltas left out: the library's host code estimates them with TORTOISE's
+    heuristic (src/qspace.rs:100-112, 209-227), the same numbers as the
+    oracle (whose estimate the CPU tests pin by b = 2 gamma^2 G^2 delta^3),
+    reports the source, and fits with them."""
+    gt, data, mask = _small_volume()
+    out = sf.shore_fit(data, gt.bvals, gt.bvecs, mask, radial_order=4, alpha_mode="fixed", alpha=1e-3,
+                       max_iter=200, tol=0.0)
+    meta = out["sidecar"]
+    big, small = orc.estimate_deltas_tortoise(gt.bvals, 80e-3)
+    assert meta["delta_source"] == "estimated_tortoise"
+    assert meta["big_delta_seconds"] == pytest.approx(big, rel=1e-15)
+    assert meta["small_delta_seconds"] == pytest.approx(small, rel=1e-15)
+    assert meta["tau_seconds"] == pytest.approx(big - small / 3.0, rel=1e-14)
+    A = orc.design_matrix(4, 700.0, gt.bvals, gt.bvecs, big, small)
+    ref = orc.fit_volume(data, mask, A, alpha_mode="fixed", alpha=1e-3, max_iter=200, tol=0.0)
+    # FP32 against FP64 over 200 iterations of this design (tau from the
+    # estimated deltas): measured median 2e-6, max 1.9e-5
+    err = _rel(out["coefficients"][mask].astype(float), ref["coefficients"][mask].astype(float))
+    nz = np.linalg.norm(ref["coefficients"][mask], axis=-1) > 0
+    assert np.median(err[nz]) <= 5e-6 and err[nz].max() <= 5e-5

@@ -138,6 +138,30 @@ def test_qvals_and_deltas(orc):
     assert 5e-3 < small < 30e-3
 
 
+def test_tortoise_deltas_physics(orc):
+    """TORTOISE's (Delta, delta) heuristic (src/qspace.rs:209-227) pinned by the
+    relation it encodes rather than by its own arithmetic: in SI units
+    (b in s/m^2 = 1e6 x s/mm^2, gamma in rad/s/T, G in T/m) the estimate
+    satisfies b = 2 gamma^2 G^2 delta^3 with Delta = 3 delta; it scales as
+    b^(1/3) and G^(-2/3); b = 3000 s/mm^2 at 80 mT/m gives 14.85 ms (the
+    reference test's 13-15 ms range, src/qspace.rs:251-266); with no
+    diffusion weighting it falls back to delta = 1 ms, Delta = 3 ms."""
+    gamma = 267.51532e6  # src/qspace.rs:18
+    for b in (300.0, 1000.0, 3000.0, 10000.0):
+        for g in (40e-3, 80e-3, 300e-3):
+            big, small = orc.estimate_deltas_tortoise([0.0, b, 0.5 * b], g)
+            assert 2.0 * gamma ** 2 * g ** 2 * small ** 3 == pytest.approx(b * 1e6, rel=1e-13)
+            assert big == pytest.approx(3.0 * small, rel=1e-15)
+    s1 = orc.estimate_deltas_tortoise([1000.0], 80e-3)[1]
+    assert orc.estimate_deltas_tortoise([8000.0], 80e-3)[1] == pytest.approx(2.0 * s1, rel=1e-13)
+    assert orc.estimate_deltas_tortoise([1000.0], 640e-3)[1] == pytest.approx(s1 / 4.0, rel=1e-13)
+    # only the largest b-value counts
+    assert orc.estimate_deltas_tortoise([0.0, 5.0, 1000.0, 200.0], 80e-3)[1] == s1
+    small = orc.estimate_deltas_tortoise([0.0, 3000.0], 80e-3)[1]
+    assert small == pytest.approx(14.8504e-3, abs=1e-7)
+    assert orc.estimate_deltas_tortoise([0.0, 0.0], 80e-3) == pytest.approx((3e-3, 1e-3), rel=1e-15)
+
+
 def test_shells(orc):
     # src/qspace.rs:318-354 and src/fit.rs:615-617
     s = orc.dwi_shells([0.0, 1000.0, 1010.0, 990.0, 0.0, 3000.0, 3000.0])
