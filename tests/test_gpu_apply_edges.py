@@ -1,0 +1,87 @@
+"""Edge cases of the LDS-staged apply kernels, on the device through the C ABI.
+
+`tikhonov_lds_kernel` stages P and A by chunks of 16 measurements, runs the
+c = P s pass over an even number of chunks (a zero chunk after P's last when
+the count is odd) and keeps the signals in registers for up to 18 chunks
+(N <= 288); longer signals go to the kernels that read them twice.
+`epilogue_lds_kernel` (residual, R^2, BIC of the persistent solvers) stages
+A the same way with no bound on N. The cases cover one chunk (N <= 16), a
+chunk plus one sample, odd and even chunk counts, the largest register case
+and the fallback past it.
+
+Tolerances: Tikhonov coefficients and residuals against the FP64 oracle
+(TikhonovSolver::fit, src/solver/tikhonov.rs:59-76) at 1e-4 relative, as the
+order-8 tests; the epilogue's residual, R^2 and BIC against the same maps
+recomputed in FP64 (src/fit.rs:120-128, src/solver/alpha.rs:99-103) from the
+device's own coefficients, so the check isolates the epilogue from the
+solver: residual 1e-5 relative (FP32 products accumulated over N samples).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import cs_dmri_amd as sf  # noqa: E402
+from cs_dmri_amd import workloads as W  # noqa: E402
+
+from .fixtures import shell_gtab  # noqa: E402
+
+
+def _rel(c, ref):
+    return np.linalg.norm(c - ref, axis=-1) / np.maximum(np.linalg.norm(ref, axis=-1), 1e-30)
+
+
+@pytest.mark.parametrize("N,order", [(8, 2), (16, 2), (17, 2), (33, 4), (48, 4), (271, 4), (288, 4), (289, 4),
+                                     (400, 4)])
+def test_tikhonov_batch_chunk_counts(orc, N, order):
+    rng = np.random.default_rng(N)
+    K = orc.n_coeffs(order)
+    A = rng.standard_normal((N, K))
+    dn, dl = orc.regularization(order)
+    S = (A @ rng.standard_normal((K, 70))).T + 0.1 * rng.standard_normal((70, N))  # 70 voxels: partial tiles
+    C, R = sf.tikhonov_batch(A, S, dn, dl, 1e-2, 1e-2)
+    Cr, Rr = orc.tikhonov(A, S.astype(np.float32).astype(np.float64), dn, dl, 1e-2, 1e-2)
+    err = _rel(C.astype(np.float64), Cr)
+    assert err.max() <= 1e-4, (N, err.max())
+    np.testing.assert_allclose(R, Rr, rtol=1e-4)
+
+
+def _gtab(n_total):
+    """1 b0 plus golden-spiral shells with n_total measurements in all."""
+    per = (n_total - 1) // 3
+    bvals, bvecs = shell_gtab(per, shells=(1000.0, 2000.0, 3000.0))
+    extra = n_total - len(bvals)
+    if extra:
+        b2, v2 = shell_gtab(extra, shells=(2500.0,))
+        bvals = np.concatenate([bvals, b2[1:]])
+        bvecs = np.concatenate([bvecs, v2[1:]])
+    assert len(bvals) == n_total
+    return bvals, bvecs
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+@pytest.mark.parametrize("N,order", [(16, 2), (17, 2), (49, 4), (301, 6)])
+def test_epilogue_maps_chunk_counts(orc, precision, N, order):
+    bvals, bvecs = _gtab(N)
+    big, small = 0.04, 0.012
+    gt = W.GradientTable(bvals, bvecs, big, small, 50.0)
+    shape = (5, 4, 3)
+    mask = np.ones(shape, bool)
+    mask[1, 1, 1] = False
+    data = np.zeros(shape + (N,), np.float32)
+    data[mask] = W.multi_tensor_signal(gt, int(mask.sum()), W.S0 / 30, seed=N)
+    out = sf.shore_fit(data, bvals, bvecs, mask, big_delta=big, small_delta=small, radial_order=order,
+                       alpha_mode="fixed", alpha=1e-3, max_iter=150, tol=1e-6, precision=precision,
+                       diagnostics=True)
+    A = orc.design_matrix(order, 700.0, bvals, bvecs, big, small)
+    c = out["coefficients"][mask].astype(np.float64)
+    s = data[mask].astype(np.float64)
+    rss = ((s - c @ A.T) ** 2).sum(-1)
+    sst = ((s - s.mean(-1, keepdims=True)) ** 2).sum(-1)
+    np.testing.assert_allclose(out["residual_l2"][mask], np.sqrt(rss), rtol=1e-5)
+    np.testing.assert_allclose(out["r2"][mask], 1.0 - rss / sst, rtol=0, atol=1e-5)
+    nnz = (out["coefficients"][mask] != 0).sum(-1)
+    bic = N * np.log(np.maximum(rss, np.finfo(float).tiny) / N) + np.log(N) * nnz
+    np.testing.assert_allclose(out["bic"][mask], bic, rtol=1e-5, atol=1e-2)
+    assert not out["residual_l2"][~mask].any() and not out["coefficients"][~mask].any()

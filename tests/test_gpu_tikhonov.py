@@ -5,7 +5,8 @@ Cartesian q-points (seed 3, bmax 5000), SHORE order 8 (K = 145), zeta 700,
 lambda_n = lambda_l = 1e-2, multi-tensor signals at SNR 20. At this size the
 device runs the blocked Cholesky with a one-column remainder (145 = 18*8 + 1),
 the 16-lane shuffle column solve for P = (A^T A + Lambda)^-1 A^T, and the
-two-tile apply kernel (padded K 160 = 10 blocks of 16).
+LDS-staged apply kernel (padded K 160 = 10 blocks of 16; 258 samples = 17
+chunks of 16, an odd count).
 
 Tolerances
 ----------
@@ -98,7 +99,7 @@ def test_tikhonov_batch_order8(orc, cfg2):
     print(f"tikhonov_batch K=145: coef err max {err.max():.2e}")
     assert err.max() <= 1e-4
     np.testing.assert_allclose(R, Rr, rtol=1e-4)
-    # one signal, and 17 (a partial tile of each of the two tiles a wave owns)
+    # one signal, and partial 16-voxel tiles and workgroups
     for n in (1, 17, 33):
         Cn, Rn = sf.tikhonov_batch(A, S[:n], dn, dl, LAM, LAM)
         np.testing.assert_array_equal(Cn, C[:n])
