@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "voxel fits/s (SHORE-8 FISTA, coef err≤1e-4 vs FP64) + % MFMA/HBM roofline"
 PEAK = {"fp32": 157.3, "fp16": 2500.0, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNEL = {"fp32": "fista_f32_persist_kernel<10,8,true>",
           "fp16": "fista_mixed_kernel<10,8,false,true,false,false,false,145>",
           "bf16": "fista_mixed_kernel<10,8,true,true,false,false,false,145>"}
@@ -256,8 +257,8 @@ def io_bench(args):
         nbytes = data.size * (2 + 4)
         res["value"] = round(nbytes / k_ms / 1e6, 1)
         res["ms_per_step"] = round(k_ms, 4)
-        res["roofline"] = {"bound": "hbm", "achieved": res["value"], "peak": 8000.0, "unit": "GB/s",
-                           "frac": round(nbytes / k_ms / 1e6 / 8000.0, 4), "traffic": None,
+        res["roofline"] = {"bound": "hbm", "achieved": res["value"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(nbytes / k_ms / 1e6 / HBM_PEAK_GBS, 4), "traffic": None,
                            "kernel": "to_voxel_major_kernel<int16_t>", "kernel_ms": round(k_ms, 4),
                            "algorithmic_bytes": nbytes}
         want = (q.astype(np.float32) * np.float32(slope) + np.float32(inter)).transpose(3, 2, 1, 0)
@@ -295,10 +296,12 @@ def tikhonov_bench(args):
     Tikhonov L2 solver, lambda_N = lambda_L = 1e-2 (src/solver/tikhonov.rs:26-76).
     A step is one whole ``shorefit_fit``: design, G, Cholesky of G + Lambda,
     P = (G + Lambda)^-1 A^T (FP64, on the device), then c = P s and the
-    residual per voxel on FP32 MFMA (``tikhonov_lds_kernel``, signals read
-    once). The roofline is that kernel's: 4 N K algorithmic flops per voxel
-    (P s and A c) against the FP32 MFMA peak; its compulsory bytes,
-    4 (N + K + 1) per voxel, are reported beside it."""
+    residual per voxel (``tikhonov_h16_kernel``: FP16 MFMA on hi/lo splits of
+    both operands, FP32 accumulation, signals read once). The roofline is
+    that kernel's compulsory HBM bytes, 4 (N + K + 1) per voxel (signals in,
+    coefficients and residual out), against 8 TB/s; the FP16 MFMA flops it
+    executes (3 products per split pair, padded shapes) are reported beside
+    it against the FP16 peak."""
     import torch
 
     from cs_dmri_amd import _lib as L
@@ -341,19 +344,24 @@ def tikhonov_bench(args):
     kern_s = float(np.mean(kernel_ms)) * 1e-3
     flops = 4.0 * N * K * V
     nbytes = 4.0 * (N + K + 1) * V
+    Kp, Np2 = 16 * ((K + 15) // 16), 288  # padded shapes of the apply kernel
+    exec_flops = 3 * 2.0 * (2 * Kp * Np2) * V
     result = {
         "metric": "voxel fits/s (SHORE-8 Tikhonov L2)", "value": round(V * args.steps / wall, 1),
         "unit": "voxel-fits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded multi-tensor + Rician phantom)",
+        "vs_baseline": None, "dtype": "fp16 hi/lo split, fp32 accumulation",
+        "data": "synthetic (seeded multi-tensor + Rician phantom)",
         "config": {"workload": "shore8-cartesian258-96x96x60-ellipsoid", "voxels": V, "n_grads": N, "n_coeffs": K,
                    "radial_order": 8, "solver": "tikhonov-l2 lambda_n=lambda_l=1e-2"},
-        "roofline": {"bound": "mfma", "achieved": round(flops / kern_s / 1e12, 3), "peak": PEAK["fp32"],
-                     "unit": "TFLOP/s", "frac": round(flops / kern_s / 1e12 / PEAK["fp32"], 4),
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / kern_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(nbytes / kern_s / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": traffic_bytes("tikhonov"),
-                     "kernel": "tikhonov_lds_kernel<10,18,4>", "kernel_ms": round(kern_s * 1e3, 3),
-                     "flops_per_launch": flops, "algorithmic_bytes": nbytes,
-                     "GB_s": round(nbytes / kern_s / 1e9, 1)},
+                     "kernel": "tikhonov_h16_kernel<10,18,4>", "kernel_ms": round(kern_s * 1e3, 3),
+                     "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
+                     "executed": {"mfma_flops_per_launch": exec_flops,
+                                  "tflops": round(exec_flops / kern_s / 1e12, 3),
+                                  "frac": round(exec_flops / kern_s / 1e12 / PEAK["fp16"], 4)}},
     }
     if not args.no_cpu:
         from oracle import load

@@ -464,10 +464,20 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
 }
 
 // P = (A^T A + lambda_n D_n + lambda_l D_l)^-1 A^T in FP64 (Cholesky on the
-// device), packed as the FP32 prologue operand.
+// device), packed as the FP32 prologue operand and, where the split-FP16
+// apply takes the shape, as its hi / lo chunks.
 struct TikOps {
     DevBuf<float> P, A;
+    DevBuf<uint16_t> H16;
+    DevBuf<float> inv16;
 };
+
+void set_tik_ops(TikhonovLaunch& L, const TikOps& ops) {
+    L.Pfrag = ops.P.p;
+    L.Afrag = ops.A.p;
+    L.H16 = ops.H16.p;
+    L.inv16 = ops.inv16.p;
+}
 
 void pack_tikhonov(const Design& d, const double* dn, const double* dl, double lam_n, double lam_l, TikOps& ops,
                    hipStream_t s);
@@ -523,6 +533,15 @@ void pack_tikhonov(const Design& d, const double* dn, const double* dl, double l
     ops.A.alloc(size_t(d.Np) * d.Kp);
     launch_pack_rowblock(P.p, d.K, d.N, d.N, /*transpose=*/false, 1.0, d.Kp, d.Np, ops.P.p, s);
     launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
+    const int MB = d.Kp / 16, NC = d.Np / 16;
+    if (MB <= kTik16MaxMB && NC <= kTik16MaxBlocks) {
+        // the apply kernel's fixed chunk counts: G2 chunks of two sample
+        // blocks, G1 (even) chunks of P, two zero chunks after them
+        const int G2 = kTik16MaxBlocks / 2, G1 = (G2 + 1) & ~1;
+        ops.H16.alloc(size_t(G1 + G2 + 2) * 4 * ((MB + 1) / 2) * 64 * 8);
+        ops.inv16.alloc(size_t(d.Kp) + 32 * G2);
+        launch_pack_tik16(P.p, d.A.p, d.K, d.N, MB, G1, G2, ops.inv16.p, ops.H16.p, s);
+    }
     SF_HIP(hipStreamSynchronize(s));
 }
 
@@ -996,8 +1015,7 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
             T.K = d.K;
             T.Kp = d.Kp;
             T.Np = d.Np;
-            T.Pfrag = tik.P.p;
-            T.Afrag = tik.A.p;
+            set_tik_ops(T, tik);
             T.rss_l2 = rssl2;
             launch_tikhonov(T, s);
         }
@@ -1038,8 +1056,7 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
         L.K = d.K;
         L.Kp = d.Kp;
         L.Np = d.Np;
-        L.Pfrag = ops.P.p;
-        L.Afrag = ops.A.p;
+        set_tik_ops(L, ops);
         L.coef = coef;
         L.resid = res;
         L.r2 = r2;
@@ -1426,8 +1443,7 @@ int shorefit_tikhonov_batch(const double* A, int32_t n_rows, int32_t n_cols, con
         L.K = d.K;
         L.Kp = d.Kp;
         L.Np = d.Np;
-        L.Pfrag = ops.P.p;
-        L.Afrag = ops.A.p;
+        set_tik_ops(L, ops);
         L.coef = C.p;
         L.resid = R.p;
         launch_tikhonov(L, s);

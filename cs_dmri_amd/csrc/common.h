@@ -127,6 +127,16 @@ void launch_pack_rowblock(const double* d_src, int rows, int cols, int src_ld, b
 // part 2 the remainder unscaled in the same type (split increments).
 void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, uint16_t* d_frag, hipStream_t s,
                         int part = 0);
+// Split-FP16 operands of the Tikhonov apply (P: K x N, A: N x K, FP64, row
+// major): per-row inverse scales (16 MB rows of P, then 32 G2 rows of A) and
+// G1e + G2 + 2 chunks of 4 ceil(MB / 2) x 64 FP16 fragments (8 halves a
+// lane; the last two are zeros).
+void launch_pack_tik16(const double* d_P, const double* d_A, int K, int N, int MB, int G1e, int G2, float* d_inv,
+                       uint16_t* d_frag, hipStream_t s);
+// The split-FP16 apply serves padded coefficient counts up to 16 kTik16MaxMB
+// and up to 16 kTik16MaxBlocks measurements (its signals stay in registers).
+constexpr int kTik16MaxMB = 10;
+constexpr int kTik16MaxBlocks = 18;
 // dst (rows x cols_p, FP64) = src (rows x cols) zero-padded on the right.
 void launch_pad_cols(const double* d_src, int rows, int cols, int cols_p, double* d_dst, hipStream_t s);
 
@@ -227,6 +237,8 @@ struct TikhonovLaunch {
     int K, Kp, Np;
     const float* Pfrag;  // (G + Lambda)^-1 A^T, Kp x Np fragment-major
     const float* Afrag;
+    const uint16_t* H16;  // split-FP16 chunks of P and A (null: the FP32 apply)
+    const float* inv16;   // their row scales' inverses
     float* coef;
     float* resid;
     float* r2;

@@ -901,27 +901,6 @@ void launch_epilogue(const FistaArgs& args, const float* X, const uint32_t* its,
 }
 
 template <int MB>
-void launch_tik(const FistaArgs& args, int64_t n_vox, float* rss_l2, hipStream_t s) {
-    const int64_t tiles = (n_vox + 15) / 16;
-    if (args.NC <= kTikLdsChunks) {
-        constexpr int W = 4;
-        hipLaunchKernelGGL((tikhonov_lds_kernel<MB, kTikLdsChunks, W>), dim3(unsigned((tiles + W - 1) / W)),
-                           dim3(64 * W), 0, s, args, rss_l2);
-        SF_HIP(hipGetLastError());
-        return;
-    }
-    if constexpr (MB >= 8 && MB <= 10) {
-        constexpr int T = 2;
-        hipLaunchKernelGGL((tikhonov_multi_kernel<MB, T>), dim3(unsigned((tiles + 4 * T - 1) / (4 * T))), dim3(256),
-                           0, s, args, rss_l2);
-        SF_HIP(hipGetLastError());
-        return;
-    }
-    hipLaunchKernelGGL(tikhonov_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, args, rss_l2);
-    SF_HIP(hipGetLastError());
-}
-
-template <int MB>
 void launch_rhs(const FistaLaunch& L, hipStream_t s) {
     constexpr int VPL = 2;
     const unsigned grid = unsigned((L.n_vox + 64 * VPL - 1) / (64 * VPL));
@@ -1271,6 +1250,356 @@ __device__ __forceinline__ float pow2_to(float bound, int target) {
     int e = 0;
     (void)frexpf(bound, &e);
     return ldexpf(1.f, target - e);
+}
+
+// ---------------------------------------------------------------------------
+// Tikhonov L2 apply on the FP16 MFMA (src/solver/tikhonov.rs:59-76), c = P s
+// and |s - A c|^2 with the signals read once.
+//
+// Each operand x is carried as two FP16 numbers: x' = x 2^e scaled by a power
+// of two into [2^14, 2^15) in magnitude (per row of P and of A, per voxel for
+// s and for c), hi = fp16(x') and lo = fp16(x' - hi), and each product as
+// hi*hi + hi*lo + lo*hi on one FP32 accumulation chain: the representation
+// keeps ~22 bits and the dropped lo*lo term is 2^-22 of the product, so the
+// error is of the order of FP32 arithmetic's (where lo is subnormal, below
+// ~2^-3 in scaled units, it keeps 2^-25 of the scaled unit in absolute terms).
+// Three 16x16x32 FP16 MFMAs replace eight 16x16x4 FP32 ones per 32 products,
+// about 4.4x fewer matrix-core cycles.
+//
+// Pass 1 runs over chunks of two 16-sample blocks: the signals, scaled and
+// split, are the B operand (lane (h, j) holds voxel j's samples 16 nb + 4h +
+// r of blocks nb = 2g, 2g + 1, the layout the residual needs), P's fragments
+// the A operand. Pass 2 runs over the same chunks: A's rows of the two
+// sample blocks against c, scaled and split, as the B operand (c in the
+// accumulator layout of pass 1: coefficients 16 mb + 4h + r, blocks 2 kc and
+// 2 kc + 1 per k-chunk). The chunks are staged in LDS for the W waves of a
+// workgroup as in the FP32 kernel (double buffered, requested from L2 two
+// chunks ahead). Chunk counts are those of the largest signal (NCM sample
+// blocks), zeros past the actual one, and pass 1 runs over an even number of
+// them, so that chunk parities are compile-time constants.
+struct Tik16Args {
+    const float* data;
+    const int64_t* vox;
+    int64_t n_vox;
+    int64_t sig_stride;
+    int N, K, NC;
+    const floatx4* H;    // G1 + G2 + 2 chunks (G2 = 9 for 18 sample blocks, G1 = 10) of 4 ceil(MB / 2) x 64 fragments
+    const float* inv;    // inverse row scales: 16 MB rows of P, then the rows of A
+    float* coef;
+    float* resid;
+    float* r2;
+    uint32_t* iters;
+    uint8_t* kind;
+    float* rss_l2;
+};
+
+// (fragments as floatx4: a native vector type, which the compiler keeps in
+// registers where HIP's uint4 struct made it stage the set in scratch)
+// s_buffer resource word 3 for raw (untyped) buffer loads on gfx9 parts
+constexpr int kBufferWord3 = 0x00020000;
+
+template <int NF, int NT>
+struct FlatStage {
+    static constexpr int PER = (NF + NT - 1) / NT;
+    floatx4 pre[2][PER];
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t off[PER];  // byte offsets of this thread's fragments in a chunk
+    // Buffer loads: the chunk offset goes in the scalar offset and the lane
+    // offsets stay fixed, so a chunk request costs no vector address
+    // arithmetic (a global load takes a 64-bit address per lane).
+    __device__ __forceinline__ void init(const floatx4* base, int n_chunks, int tid) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<floatx4*>(base), 0, n_chunks * NF * 16, kBufferWord3);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) off[i] = uint32_t(min(tid + i * NT, NF - 1)) * 16u;
+    }
+    template <int SET>
+    __device__ __forceinline__ void load(int chunk) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+            pre[SET][i] = __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off[i],
+                                                                                            chunk * NF * 16, 0));
+    }
+    template <int SET>
+    __device__ __forceinline__ void store(floatx4* buf, int tid) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + i * NT;
+            if (NF % NT == 0 || e < NF) buf[e] = pre[SET][i];
+        }
+    }
+};
+
+// eight scaled values as the hi and lo halves of an FP16 B operand
+__device__ __forceinline__ void split8(const float (&t)[8], uint4& hi, uint4& lo) {
+    uint32_t hv[4], lv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        hv[q] = pack2<false>(t[2 * q], t[2 * q + 1]);
+        const float2v b = unpack2<false>(hv[q]);
+        lv[q] = pack2<false>(t[2 * q] - b[0], t[2 * q + 1] - b[1]);
+    }
+    hi = uint4{hv[0], hv[1], hv[2], hv[3]};
+    lo = uint4{lv[0], lv[1], lv[2], lv[3]};
+}
+
+template <int MB, int NCM, int W>
+__global__ void __launch_bounds__(64 * W, 2) tikhonov_h16_kernel(const Tik16Args a) {  // two waves a SIMD
+    static_assert(NCM % 2 == 0, "chunks of two sample blocks");
+    constexpr int MB2 = (MB + 1) / 2;       // coefficient k-chunks of pass 2
+    constexpr int NF = 4 * MB2 * 64;        // fragments per chunk (pass 1 uses 2 MB x 64)
+    constexpr int GM = NCM / 2;             // pass-2 chunks
+    constexpr int G1M = (GM + 1) & ~1;      // pass-1 chunks (even; zeros past the signal)
+    constexpr int NR = 16 * MB + 32 * GM;   // row scales
+    constexpr int NT = 64 * W;
+    __shared__ floatx4 fb[2][NF];
+    __shared__ float rs[NR];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    const int64_t v = (int64_t(blockIdx.x) * W + wave) * 16 + j;
+    const bool valid = v < a.n_vox;
+    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
+    const float* sig = a.data + out_idx * a.sig_stride;
+    const int NC = a.NC;                    // <= NCM, checked by the host
+    // Fixed chunk counts and unconditional staging loads (the packer adds
+    // two zero chunks at the end): a load issued under a branch makes the
+    // compiler wait for every outstanding load where the paths join, which
+    // would expose the L2 latency of the chunk requested two steps ahead.
+    FlatStage<NF, NT> st;
+    st.init(a.H, G1M + GM + 2, tid);
+    st.template load<0>(0);
+    st.template load<1>(1);
+#pragma unroll
+    for (int i = 0; i < (NR + NT - 1) / NT; ++i) {
+        const int e = tid + i * NT;
+        if (NR % NT == 0 || e < NR) rs[e] = a.inv[e];
+    }
+    // every signal load is issued here (unguarded: rows past N in the last
+    // block read sample N - 1, meet zero columns of P and are left out of
+    // the sums)
+    float sreg[NCM][4];
+    const float* sb = sig + 4 * h;
+#pragma unroll
+    for (int ch = 0; ch < NCM; ++ch) {
+        if (ch + 1 < NC) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sreg[ch][r] = sb[16 * ch + r];
+        } else if (ch + 1 == NC) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sreg[ch][r] = sig[min(16 * ch + 4 * h + r, a.N - 1)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sreg[ch][r] = 0.f;
+        }
+    }
+    st.template store<0>(fb[0], tid);
+    // rows past N become zeros once here (every load has arrived: the scale
+    // needs them all), so that no sum below needs a per-row mask
+    float smax = 0.f, sum = 0.f;
+#pragma unroll
+    for (int ch = 0; ch < NCM; ++ch)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (ch + 1 >= NC) sreg[ch][r] = (16 * ch + 4 * h + r < a.N) ? sreg[ch][r] : 0.f;
+            smax = fmaxf(smax, fabsf(sreg[ch][r]));
+            sum += sreg[ch][r];
+        }
+    const float ss = pow2_to(lane_max4(smax), 15);
+    __syncthreads();
+    // one step of the chunk pipeline: request chunk g + 2 into the register
+    // set chunk g came from, multiply chunk g, store chunk g + 1
+    // (scheduling fences keep the requests at the start of the step and the
+    // LDS stores at its end: left to itself the compiler sinks the requests
+    // towards their stores, one step before the data is needed)
+    auto step_begin = [&](int g, auto par) __attribute__((always_inline)) {
+        constexpr int P = decltype(par)::value;
+        st.template load<P>(g + 2);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto step_end = [&](auto par) __attribute__((always_inline)) {
+        constexpr int P = decltype(par)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        st.template store<1 - P>(fb[1 - P], tid);
+        __syncthreads();
+    };
+    floatx4 c[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) c[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < G1M; ++g) {
+        if (g & 1) step_begin(g, std::integral_constant<int, 1>{});
+        else step_begin(g, std::integral_constant<int, 0>{});
+        if (g < GM) {  // compile-time: the last pass-1 chunk may be padding
+            float t[8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                t[r] = sreg[2 * g < NCM ? 2 * g : 0][r] * ss;
+                t[4 + r] = sreg[2 * g + 1 < NCM ? 2 * g + 1 : 0][r] * ss;
+            }
+            uint4 bh, bl;
+            split8(t, bh, bl);
+            const floatx4* f = fb[g & 1];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const uint4 fh = __builtin_bit_cast(uint4, f[(2 * mb) * 64 + lane]);
+                const uint4 fl = __builtin_bit_cast(uint4, f[(2 * mb + 1) * 64 + lane]);
+                c[mb] = mfma16<false>(fh, bh, c[mb]);
+                c[mb] = mfma16<false>(fh, bl, c[mb]);
+                c[mb] = mfma16<false>(fl, bh, c[mb]);
+            }
+            // two blocks of fragments in flight from LDS ahead of the MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x100, MB < 2 ? 2 * MB : 4, 0);
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+                if (mb + 2 < MB) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+            }
+        }
+        if (g & 1) step_end(std::integral_constant<int, 1>{});
+        else step_end(std::integral_constant<int, 0>{});
+    }
+    // c = c' / (row scale x ss), written out; then scaled and split as the
+    // B operand of pass 2
+    const float iss = 1.f / ss;
+    float cm = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            c[mb][r] *= rs[16 * mb + 4 * h + r] * iss;
+            cm = fmaxf(cm, fabsf(c[mb][r]));
+        }
+    if (valid && a.coef)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * mb + 4 * h + r;
+                if (k < a.K) a.coef[out_idx * a.K + k] = c[mb][r];
+            }
+    const float cs = pow2_to(lane_max4(cm), 15), ics = 1.f / cs;
+    uint4 chh[MB2], cll[MB2];
+#pragma unroll
+    for (int kc = 0; kc < MB2; ++kc) {
+        float t[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            t[r] = c[2 * kc][r] * cs;
+            t[4 + r] = 2 * kc + 1 < MB ? c[2 * kc + 1 < MB ? 2 * kc + 1 : 0][r] * cs : 0.f;
+        }
+        split8(t, chh[kc], cll[kc]);
+    }
+    const float mean = lane_sum4(sum) / float(a.N);
+    float lastm[4];  // rows of the last sample block inside the signal
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lastm[r] = 16 * (NC - 1) + 4 * h + r < a.N ? 1.f : 0.f;
+    float rss = 0.f, sst = 0.f;
+    const float* ars = rs + 16 * MB;
+#pragma unroll
+    for (int g2 = 0; g2 < GM; ++g2) {
+        const int g = G1M + g2;  // same parity as g2
+        if (g2 & 1) step_begin(g, std::integral_constant<int, 1>{});
+        else step_begin(g, std::integral_constant<int, 0>{});
+        const floatx4* f = fb[g2 & 1];
+        floatx4 ac[2];  // sample blocks 2 g2 and 2 g2 + 1, two chains
+#pragma unroll
+        for (int b = 0; b < 2; ++b) ac[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < MB2; ++kc)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const uint4 fh = __builtin_bit_cast(uint4, f[(2 * (b * MB2 + kc)) * 64 + lane]);
+                const uint4 fl = __builtin_bit_cast(uint4, f[(2 * (b * MB2 + kc) + 1) * 64 + lane]);
+                ac[b] = mfma16<false>(fh, chh[kc], ac[b]);
+                ac[b] = mfma16<false>(fh, cll[kc], ac[b]);
+                ac[b] = mfma16<false>(fl, chh[kc], ac[b]);
+            }
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+        for (int kc = 0; kc < MB2; ++kc) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+            if (kc + 1 < MB2) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+        }
+        // rows past N: zero signal against zero rows of A, so e = 0; blocks
+        // past the signal are skipped and the last one masks x - mean
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int nb = 2 * g2 + b;
+            if (nb < NC) {  // uniform
+                float dm[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float x = sreg[nb][r];
+                    const float e = fmaf(-ac[b][r], ars[16 * nb + 4 * h + r] * ics, x);
+                    rss = fmaf(e, e, rss);
+                    dm[r] = x - mean;
+                }
+                if (nb + 1 == NC)  // uniform
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) dm[r] *= lastm[r];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sst = fmaf(dm[r], dm[r], sst);
+            }
+        }
+        if (g2 & 1) step_end(std::integral_constant<int, 1>{});
+        else step_end(std::integral_constant<int, 0>{});
+    }
+    rss = lane_sum4(rss);
+    sst = lane_sum4(sst);
+    if (!valid || h != 0) return;
+    const double ss_res = double(rss);
+    if (a.resid) a.resid[out_idx] = float(sqrt(ss_res));
+    if (a.r2) a.r2[out_idx] = float(sst > 0.f ? 1.0 - ss_res / double(sst) : 0.0);
+    if (a.iters) a.iters[out_idx] = 1;
+    if (a.kind) a.kind[out_idx] = 2;
+    if (a.rss_l2) a.rss_l2[out_idx] = float(ss_res);
+}
+
+template <int MB>
+void launch_tik(const FistaArgs& args, const TikhonovLaunch& L, hipStream_t s) {
+    const int64_t n_vox = L.n_vox;
+    const int64_t tiles = (n_vox + 15) / 16;
+    if constexpr (MB <= kTik16MaxMB) {
+        if (L.H16 && args.NC <= kTik16MaxBlocks) {
+            constexpr int W = 4;
+            Tik16Args t{};
+            t.data = args.data;
+            t.vox = args.vox;
+            t.n_vox = n_vox;
+            t.sig_stride = args.sig_stride;
+            t.N = args.N;
+            t.K = args.K;
+            t.NC = args.NC;
+            t.H = reinterpret_cast<const floatx4*>(L.H16);
+            t.inv = L.inv16;
+            t.coef = args.coef;
+            t.resid = args.resid;
+            t.r2 = args.r2;
+            t.iters = args.iters;
+            t.kind = args.kind;
+            t.rss_l2 = L.rss_l2;
+            hipLaunchKernelGGL((tikhonov_h16_kernel<MB, kTik16MaxBlocks, W>), dim3(unsigned((tiles + W - 1) / W)),
+                               dim3(64 * W), 0, s, t);
+            SF_HIP(hipGetLastError());
+            return;
+        }
+    }
+    float* rss_l2 = L.rss_l2;
+    if (args.NC <= kTikLdsChunks) {
+        constexpr int W = 4;
+        hipLaunchKernelGGL((tikhonov_lds_kernel<MB, kTikLdsChunks, W>), dim3(unsigned((tiles + W - 1) / W)),
+                           dim3(64 * W), 0, s, args, rss_l2);
+        SF_HIP(hipGetLastError());
+        return;
+    }
+    if constexpr (MB >= 8 && MB <= 10) {
+        constexpr int T = 2;
+        hipLaunchKernelGGL((tikhonov_multi_kernel<MB, T>), dim3(unsigned((tiles + 4 * T - 1) / (4 * T))), dim3(256),
+                           0, s, args, rss_l2);
+        SF_HIP(hipGetLastError());
+        return;
+    }
+    hipLaunchKernelGGL(tikhonov_kernel<MB>, dim3(unsigned((tiles + 3) / 4)), dim3(256), 0, s, args, rss_l2);
+    SF_HIP(hipGetLastError());
 }
 
 // SPLITINC: the increments apply both halves of the split operator,
@@ -2061,16 +2390,16 @@ void launch_tikhonov(const TikhonovLaunch& L, hipStream_t s) {
     a.iters = L.iters;
     a.kind = L.kind;
     switch (L.Kp / 16) {
-        case 1: launch_tik<1>(a, L.n_vox, L.rss_l2, s); break;
-        case 2: launch_tik<2>(a, L.n_vox, L.rss_l2, s); break;
-        case 3: launch_tik<3>(a, L.n_vox, L.rss_l2, s); break;
-        case 4: launch_tik<4>(a, L.n_vox, L.rss_l2, s); break;
-        case 5: launch_tik<5>(a, L.n_vox, L.rss_l2, s); break;
-        case 6: launch_tik<6>(a, L.n_vox, L.rss_l2, s); break;
-        case 8: launch_tik<8>(a, L.n_vox, L.rss_l2, s); break;
-        case 10: launch_tik<10>(a, L.n_vox, L.rss_l2, s); break;
-        case 12: launch_tik<12>(a, L.n_vox, L.rss_l2, s); break;
-        case 16: launch_tik<16>(a, L.n_vox, L.rss_l2, s); break;
+        case 1: launch_tik<1>(a, L, s); break;
+        case 2: launch_tik<2>(a, L, s); break;
+        case 3: launch_tik<3>(a, L, s); break;
+        case 4: launch_tik<4>(a, L, s); break;
+        case 5: launch_tik<5>(a, L, s); break;
+        case 6: launch_tik<6>(a, L, s); break;
+        case 8: launch_tik<8>(a, L, s); break;
+        case 10: launch_tik<10>(a, L, s); break;
+        case 12: launch_tik<12>(a, L, s); break;
+        case 16: launch_tik<16>(a, L, s); break;
         default:
             throw Error{SHOREFIT_E_UNSUPPORTED,
                         "Tikhonov kernel: padded coefficient count " + std::to_string(L.Kp) + " is not instantiated"};

@@ -425,6 +425,64 @@ __global__ void pack_M_lowp_kernel(const double* __restrict__ G, int K, int Kp, 
     }
 }
 
+// Operands of the split-FP16 Tikhonov apply (tikhonov_h16_kernel). Every row
+// of P (K x N) and of A (N x K) is scaled by a power of two, 2^(15 - E) with
+// max_j |x_ij| = m 2^E, m in [0.5, 1) (1 for a zero row), so that it lies in
+// [2^14, 2^15) in magnitude; inv[] receives the inverse scales, the Kp rows
+// of P first, then the rows of A padded to Np2.
+__global__ void __launch_bounds__(64) tik16_scales_kernel(const double* __restrict__ P, const double* __restrict__ A,
+                                                          int K, int N, int Kp, float* __restrict__ inv) {
+    const int t = blockIdx.x, lane = threadIdx.x;  // one wave per row
+    const double* row = nullptr;
+    int len = 0;
+    if (t < Kp) {
+        if (t < K) row = P + size_t(t) * N, len = N;
+    } else if (t - Kp < N) {
+        row = A + size_t(t - Kp) * K, len = K;
+    }
+    double m = 0.0;
+    for (int j = lane; j < len; j += 64) m = fmax(m, fabs(row[j]));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (lane != 0) return;
+    int e = 0;
+    (void)frexp(m, &e);
+    inv[t] = m > 0.0 ? float(ldexp(1.0, e - 15)) : 1.f;
+}
+
+// The fragments, chunk by chunk: chunks g < G1e hold, for coefficient block
+// mb, slots 2 mb (hi) and 2 mb + 1 (lo) of P's rows 16 mb + i and columns
+// p(v) = 16 (2g + v/4) + 4h + v%4 for lane 16h + i and half v (the samples
+// lane (h, j) keeps in registers v%4 of sample blocks 2g and 2g + 1); chunks
+// G1e + g2 hold, for sample block 2 g2 + b (b = 0, 1) and coefficient pair
+// kc, slots 2 (b MB2 + kc) (hi) and + 1 (lo) of A's rows 16 (2 g2 + b) + i
+// and columns 16 (2 kc + v/4) + 4h + v%4 (the coefficients lane (h, j) holds
+// in registers v%4 of coefficient blocks 2 kc and 2 kc + 1). hi = fp16(x')
+// and lo = fp16(x' - hi) of the scaled value x', zero outside the matrices
+// (two chunks of zeros follow the G1e + G2: 16 (2 g2 + b) >= N there).
+__global__ void pack_tik16_kernel(const double* __restrict__ P, const double* __restrict__ A, int K, int N, int MB,
+                                  int G1e, int G2, const float* __restrict__ inv, uint16_t* __restrict__ frag) {
+    const int MB2 = (MB + 1) / 2, CF = 4 * MB2;
+    const int64_t total = int64_t(G1e + G2 + 2) * CF * 64 * 8;  // the last two chunks: zeros
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int v = int(t & 7), l = int((t >> 3) & 63);
+    const int64_t cs = t >> 9;
+    const int slot = int(cs % CF), g = int(cs / CF);
+    const int h = l >> 4, i = l & 15, hl = slot & 1;
+    double x = 0.0;
+    if (g < G1e) {
+        const int mb = slot >> 1, row = 16 * mb + i, p = 16 * (2 * g + (v >> 2)) + 4 * h + (v & 3);
+        if (mb < MB && row < K && p < N) x = P[size_t(row) * N + p] / double(inv[row]);
+    } else {
+        const int q = slot >> 1, b = q / MB2, kc = q % MB2;
+        const int n = 16 * (2 * (g - G1e) + b) + i, k = 16 * (2 * kc + (v >> 2)) + 4 * h + (v & 3);
+        if (n < N && k < K) x = A[size_t(n) * K + k] / double(inv[16 * MB + n]);
+    }
+    __half hv = __float2half(float(x));
+    if (hl) hv = __float2half(float(x - double(__half2float(hv))));
+    frag[t] = *reinterpret_cast<uint16_t*>(&hv);
+}
+
 // P = L^-T L^-1 A^T, 16 columns of P per workgroup (column n solves for row n
 // of A; P is K x N row-major). The 16 lanes of a column
 // (all in one wave) keep its vector in registers, lane u the entries
@@ -688,6 +746,17 @@ void launch_cholesky(double* d_G, int K, int* d_status, hipStream_t s) {
 void launch_pack_M(const double* d_G, int K, int Kp, double c, float* d_frag, hipStream_t s) {
     const int64_t total = int64_t(Kp) * Kp;
     hipLaunchKernelGGL(pack_M_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_G, K, Kp, c, d_frag);
+    SF_HIP(hipGetLastError());
+}
+
+void launch_pack_tik16(const double* d_P, const double* d_A, int K, int N, int MB, int G1e, int G2, float* d_inv,
+                       uint16_t* d_frag, hipStream_t s) {
+    const int Kp = 16 * MB, Np2 = 32 * G2;
+    hipLaunchKernelGGL(tik16_scales_kernel, dim3(unsigned(Kp + Np2)), dim3(64), 0, s, d_P, d_A, K, N, Kp, d_inv);
+    SF_HIP(hipGetLastError());
+    const int64_t total = int64_t(G1e + G2 + 2) * (4 * ((MB + 1) / 2)) * 64 * 8;
+    hipLaunchKernelGGL(pack_tik16_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, d_P, d_A, K, N, MB, G1e, G2,
+                       d_inv, d_frag);
     SF_HIP(hipGetLastError());
 }
 

@@ -1,13 +1,17 @@
 """Edge cases of the LDS-staged apply kernels, on the device through the C ABI.
 
-`tikhonov_lds_kernel` stages P and A by chunks of 16 measurements, runs the
-c = P s pass over an even number of chunks (a zero chunk after P's last when
-the count is odd) and keeps the signals in registers for up to 18 chunks
-(N <= 288); longer signals go to the kernels that read them twice.
-`epilogue_lds_kernel` (residual, R^2, BIC of the persistent solvers) stages
-A the same way with no bound on N. The cases cover one chunk (N <= 16), a
-chunk plus one sample, odd and even chunk counts, the largest register case
-and the fallback past it.
+`tikhonov_h16_kernel` (N <= 288, padded K <= 160) runs c = P s and the
+residual on the FP16 MFMA with hi/lo splits of both operands; it stages P and
+A by chunks of 32 measurements (two 16-sample blocks), a fixed number of
+chunks with zeros past the signal, and keeps the signals in registers; the
+last sample block of the signal is partial unless 16 divides N. Longer
+signals, and padded K above 160 (order 10), go to the FP32 kernels
+(`tikhonov_lds_kernel` up to 288 measurements, then the kernels that read the
+signals twice). `epilogue_lds_kernel` (residual, R^2, BIC of the persistent
+solvers) stages A by chunks of 16 rows with no bound on N. The cases cover
+one block (N <= 16), a block plus one sample, odd and even block counts, the
+largest register case, the fallback past it, and order 10 on both sides of
+288 measurements.
 
 Tolerances: Tikhonov coefficients and residuals against the FP64 oracle
 (TikhonovSolver::fit, src/solver/tikhonov.rs:59-76) at 1e-4 relative, as the
@@ -33,7 +37,7 @@ def _rel(c, ref):
 
 
 @pytest.mark.parametrize("N,order", [(8, 2), (16, 2), (17, 2), (33, 4), (48, 4), (271, 4), (288, 4), (289, 4),
-                                     (400, 4)])
+                                     (400, 4), (200, 10), (300, 10)])
 def test_tikhonov_batch_chunk_counts(orc, N, order):
     rng = np.random.default_rng(N)
     K = orc.n_coeffs(order)

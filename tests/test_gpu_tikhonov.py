@@ -5,20 +5,21 @@ Cartesian q-points (seed 3, bmax 5000), SHORE order 8 (K = 145), zeta 700,
 lambda_n = lambda_l = 1e-2, multi-tensor signals at SNR 20. At this size the
 device runs the blocked Cholesky with a one-column remainder (145 = 18*8 + 1),
 the 16-lane shuffle column solve for P = (A^T A + Lambda)^-1 A^T, and the
-LDS-staged apply kernel (padded K 160 = 10 blocks of 16; 258 samples = 17
-chunks of 16, an odd count).
+split-FP16 apply kernel (padded K 160 = 10 blocks of 16; 258 samples = 17
+blocks of 16, an odd count, the last one partial).
 
 Tolerances
 ----------
-* coefficients (FP64 factorisation, FP32 apply) vs the FP64 oracle
+* coefficients (FP64 factorisation; apply on FP16 hi/lo splits with FP32
+  accumulation, ~22 bits per product) vs the FP64 oracle
   (TikhonovSolver::fit, src/solver/tikhonov.rs:59-76): 1e-4 relative per voxel;
 * residual_l2: 1e-4 relative (the reference's residual is |s - A c| in FP64);
 * normal equations (tests/optimization_kkt.rs:164-218 restated on the
   device's FP32 coefficients, evaluated in FP64): the relative residual
   |(A^T A + Lambda) c - A^T s| / |A^T s| is below 1e-5, the relative
-  scale of FP32 coefficient parity (measured: at most 6.4e-7, against 5e-8 for the
-  oracle's FP64 solution merely rounded to float32; the apply c = P s
-  accumulates 258 FP32 products per coefficient). The oracle's own FP64
+  scale of FP32 coefficient parity (measured: at most 4.4e-7 with the
+  split-FP16 apply and 6.4e-7 with the FP32 one, against 5e-8 for the
+  oracle's FP64 solution merely rounded to float32). The oracle's own FP64
   solution meets the reference's 1e-10 on this design
   (tests/test_oracle_solvers.py);
 * rss_l2 map of the L2-anchored strategy (src/fit.rs:433-438): 1e-4 relative
