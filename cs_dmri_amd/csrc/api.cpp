@@ -272,6 +272,8 @@ struct FistaOps {
     DevBuf<float> M, A;
     DevBuf<double> Ap;  // FP64 design zero-padded to N x Kp
     DevBuf<uint16_t> Mlow, Mlow2, Mlow3;
+    DevBuf<uint16_t> AH16;  // A's split-FP16 chunks (epilogue residual)
+    DevBuf<float> Ainv16;
     double step = 0.0;
 };
 
@@ -421,6 +423,13 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
     ops.A.alloc(size_t(d.Np) * d.Kp);
     launch_pack_M(d.G.p, d.K, d.Kp, c, ops.M.p, s);
     launch_pack_rowblock(d.A.p, d.N, d.K, d.K, /*transpose=*/false, 1.0, d.Np, d.Kp, ops.A.p, s);
+    if (d.Kp / 16 <= kTik16MaxMB && d.Np / 16 <= kTik16MaxBlocks) {
+        // A's chunks of the split-FP16 residual pass (fixed count, two zero chunks after them)
+        const int G2 = kTik16MaxBlocks / 2;
+        ops.AH16.alloc(size_t(G2 + 2) * 4 * ((d.Kp / 16 + 1) / 2) * 64 * 8);
+        ops.Ainv16.alloc(size_t(d.Kp) + 32 * G2);
+        launch_pack_tik16(nullptr, d.A.p, d.K, d.N, d.Kp / 16, 0, G2, ops.Ainv16.p, ops.AH16.p, s);
+    }
     if (L.precision != SHOREFIT_PREC_FP32) {
         ops.Mlow.alloc(size_t(d.Kp) * d.Kp);
         launch_pack_M_lowp(d.G.p, d.K, d.Kp, c, L.precision == SHOREFIT_PREC_BF16, ops.Mlow.p, s);
@@ -435,6 +444,8 @@ void setup_fista(Design& d, FistaOps& ops, FistaWork& w, FistaLaunch& L, hipStre
     }
     L.Mfrag = ops.M.p;
     L.Afrag = ops.A.p;
+    L.AH16 = ops.AH16.p;
+    L.Ainv16 = ops.Ainv16.p;
     L.Mlow = ops.Mlow.p;
     L.Mlow2 = ops.Mlow2.p;
     L.Mlow3 = ops.Mlow3.p;

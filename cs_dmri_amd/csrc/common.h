@@ -130,7 +130,8 @@ void launch_pack_M_lowp(const double* d_G, int K, int Kp, double c, bool bf16, u
 // Split-FP16 operands of the Tikhonov apply (P: K x N, A: N x K, FP64, row
 // major): per-row inverse scales (16 MB rows of P, then 32 G2 rows of A) and
 // G1e + G2 + 2 chunks of 4 ceil(MB / 2) x 64 FP16 fragments (8 halves a
-// lane; the last two are zeros).
+// lane; the last two are zeros). With P == nullptr and G1e = 0, A's chunks
+// alone (the residual pass of the FISTA epilogue).
 void launch_pack_tik16(const double* d_P, const double* d_A, int K, int N, int MB, int G1e, int G2, float* d_inv,
                        uint16_t* d_frag, hipStream_t s);
 // The split-FP16 apply serves padded coefficient counts up to 16 kTik16MaxMB
@@ -157,6 +158,8 @@ struct FistaLaunch {
     const float* Mfrag;
     const float* ATfrag;
     const float* Afrag;
+    const uint16_t* AH16;   // A's split-FP16 chunks for the epilogue (null: the FP32 epilogue)
+    const float* Ainv16;    // their row scales' inverses
     const uint16_t* Mlow;   // FP16/BF16 increment operand (mixed precision only)
     const uint16_t* Mlow2;  // FP16 remainder of -cG x 2^12 (split refresh, FP16 only)
     const uint16_t* Mlow3;  // remainder -cG - lowp(-cG), unscaled (split increments, padded K <= 160)

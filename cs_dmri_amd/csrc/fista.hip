@@ -164,17 +164,20 @@ __device__ __forceinline__ void residual(const FistaArgs& a, const float* sig, b
     sst = lane_sum4(sst);
 }
 
+// Coefficients of one voxel (lane (h, j) holds rows 16 mb + 4h + r), stored
+// when `store`, and its non-zero count summed over the voxel's four lanes
+// (every lane takes part in the shuffles).
 template <int MB>
-__device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx, int h, const floatx4 (&x)[MB],
-                                            float rss, float sst, uint32_t its, bool conv, double alpha_v,
-                                            bool zero_voxel) {
+__device__ __forceinline__ int write_coefs(const FistaArgs& a, int64_t out_idx, int h, const floatx4 (&x)[MB],
+                                           bool store = true) {
+    if (store)
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int k = 16 * mb + 4 * h + r;
-            if (k < a.K) a.coef[out_idx * a.K + k] = x[mb][r];
-        }
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * mb + 4 * h + r;
+                if (k < a.K) a.coef[out_idx * a.K + k] = x[mb][r];
+            }
     int nnz = 0;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
@@ -182,9 +185,15 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
         for (int r = 0; r < 4; ++r) nnz += (x[mb][r] != 0.f);
     nnz += __shfl_xor(nnz, 16);
     nnz += __shfl_xor(nnz, 32);
+    return nnz;
+}
+
+// The per-voxel maps, written by lane h == 0.
+// src/fit.rs:120-128 (R^2), src/solver/alpha.rs:99-103 (BIC),
+// src/solver/alpha.rs:178-189 (all-zero voxel of a per-voxel strategy).
+__device__ __forceinline__ void write_maps(const FistaArgs& a, int64_t out_idx, int h, int nnz, float rss, float sst,
+                                           uint32_t its, bool conv, double alpha_v, bool zero_voxel) {
     if (h != 0) return;
-    // src/fit.rs:120-128 (R^2), src/solver/alpha.rs:99-103 (BIC),
-    // src/solver/alpha.rs:178-189 (all-zero voxel of a per-voxel strategy).
     const double ss_res = zero_voxel ? 0.0 : double(rss);
     const double resid_l2 = sqrt(ss_res);
     const double r2 = sst > 0.f ? 1.0 - ss_res / double(sst) : 0.0;
@@ -199,6 +208,14 @@ __device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx,
         const double rc = fmax(resid_l2 * resid_l2, 2.2250738585072014e-308);
         a.bic[out_idx] = zero_voxel ? 0.f : float(n * log(rc / n) + log(n) * double(nnz));
     }
+}
+
+template <int MB>
+__device__ __forceinline__ void write_voxel(const FistaArgs& a, int64_t out_idx, int h, const floatx4 (&x)[MB],
+                                            float rss, float sst, uint32_t its, bool conv, double alpha_v,
+                                            bool zero_voxel) {
+    const int nnz = write_coefs<MB>(a, out_idx, h, x);
+    write_maps(a, out_idx, h, nnz, rss, sst, its, conv, alpha_v, zero_voxel);
 }
 
 // Right-hand sides b_v = c A^T s_v (c = 1/(N L)) and signal means, with
@@ -891,16 +908,6 @@ __global__ void __launch_bounds__(64 * W) epilogue_lds_kernel(const FistaArgs a,
 constexpr int kTikLdsChunks = 18;
 
 template <int MB>
-void launch_epilogue(const FistaArgs& args, const float* X, const uint32_t* its, const uint8_t* status,
-                     const float* alpha_v, hipStream_t s) {
-    constexpr int W = 4;
-    const int64_t tiles = (args.n_vox + 15) / 16;
-    hipLaunchKernelGGL((epilogue_lds_kernel<MB, W>), dim3(unsigned((tiles + W - 1) / W)), dim3(64 * W), 0, s, args, X,
-                       its, status, alpha_v);
-    SF_HIP(hipGetLastError());
-}
-
-template <int MB>
 void launch_rhs(const FistaLaunch& L, hipStream_t s) {
     constexpr int VPL = 2;
     const unsigned grid = unsigned((L.n_vox + 64 * VPL - 1) / (64 * VPL));
@@ -1554,6 +1561,178 @@ __global__ void __launch_bounds__(64 * W, 2) tikhonov_h16_kernel(const Tik16Args
     if (a.rss_l2) a.rss_l2[out_idx] = float(ss_res);
 }
 
+// Epilogue of the persistent solvers with the residual pass of the split-FP16
+// Tikhonov kernel: |s - A x|^2 on the FP16 MFMA (hi/lo splits of A's scaled
+// rows and of the voxel's scaled x, hi*hi + hi*lo + lo*hi on FP32 chains),
+// A staged in LDS by chunks of two 16-sample blocks, the signals in
+// registers; then R^2, BIC and the scatter of x (src/fit.rs:120-128,
+// src/solver/alpha.rs:99-103).
+template <int MB, int NCM, int W>
+__global__ void __launch_bounds__(64 * W, 2) epilogue_h16_kernel(const FistaArgs a, const floatx4* H,
+                                                                 const float* inv, const float* X,
+                                                                 const uint32_t* its, const uint8_t* status,
+                                                                 const float* alpha_v) {
+    static_assert(NCM % 2 == 0, "chunks of two sample blocks");
+    constexpr int MB2 = (MB + 1) / 2;
+    constexpr int NF = 4 * MB2 * 64;
+    constexpr int GM = NCM / 2;
+    constexpr int NR = 16 * MB + 32 * GM;
+    constexpr int NT = 64 * W;
+    __shared__ floatx4 fb[2][NF];
+    __shared__ float rs[NR];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 4, j = lane & 15;
+    const int64_t v = (int64_t(blockIdx.x) * W + wave) * 16 + j;
+    const bool valid = v < a.n_vox;
+    const int64_t out_idx = valid ? (a.vox ? a.vox[v] : v) : 0;
+    const float* sig = a.data + out_idx * a.sig_stride;
+    const int NC = a.NC;  // <= NCM, checked by the host
+    FlatStage<NF, NT> st;
+    st.init(H, GM + 2, tid);
+    st.template load<0>(0);
+    st.template load<1>(1);
+#pragma unroll
+    for (int i = 0; i < (NR + NT - 1) / NT; ++i) {
+        const int e = tid + i * NT;
+        if (NR % NT == 0 || e < NR) rs[e] = inv[e];
+    }
+    floatx4 x[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+        x[mb] = valid ? *reinterpret_cast<const floatx4*>(X + v * a.Kp + 16 * mb + 4 * h)
+                      : floatx4{0.f, 0.f, 0.f, 0.f};
+    const float mean = valid ? a.mean[v] : 0.f;
+    // every signal load is issued here, before the products (as in the
+    // Tikhonov kernel; requesting each chunk's samples one chunk ahead left
+    // the steps waiting on HBM latency). Rows past N read sample N - 1: they
+    // meet zero rows of A and are masked out of the sums.
+    float sreg[NCM][4];
+    {
+        const float* sbp = sig + 4 * h;
+#pragma unroll
+        for (int ch = 0; ch < NCM; ++ch) {
+            if (ch + 1 < NC) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sreg[ch][r] = sbp[16 * ch + r];
+            } else if (ch + 1 == NC) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sreg[ch][r] = sig[min(16 * ch + 4 * h + r, a.N - 1)];
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sreg[ch][r] = 0.f;
+            }
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    float xm = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) xm = fmaxf(xm, fabsf(x[mb][r]));
+    const float cs = pow2_to(lane_max4(xm), 15), ics = 1.f / cs;
+    uint4 chh[MB2], cll[MB2];
+#pragma unroll
+    for (int kc = 0; kc < MB2; ++kc) {
+        float t[8];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            t[r] = x[2 * kc][r] * cs;
+            t[4 + r] = 2 * kc + 1 < MB ? x[2 * kc + 1 < MB ? 2 * kc + 1 : 0][r] * cs : 0.f;
+        }
+        split8(t, chh[kc], cll[kc]);
+    }
+    // the coefficients go out here, so that x is dead during the residual pass
+    const int nnz = write_coefs<MB>(a, out_idx, h, x, valid);
+    float lastm[4];  // rows of the last sample block inside the signal
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lastm[r] = 16 * (NC - 1) + 4 * h + r < a.N ? 1.f : 0.f;
+    st.template store<0>(fb[0], tid);
+    __syncthreads();
+    const float* ars = rs + 16 * MB;
+    float rss = 0.f, sst = 0.f;
+    auto chunk = [&](int g, auto par) __attribute__((always_inline)) {
+        constexpr int P = decltype(par)::value;
+        st.template load<P>(g + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        const floatx4* f = fb[P];
+        floatx4 ac[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) ac[b] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < MB2; ++kc)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const uint4 fh = __builtin_bit_cast(uint4, f[(2 * (b * MB2 + kc)) * 64 + lane]);
+                const uint4 fl = __builtin_bit_cast(uint4, f[(2 * (b * MB2 + kc) + 1) * 64 + lane]);
+                ac[b] = mfma16<false>(fh, chh[kc], ac[b]);
+                ac[b] = mfma16<false>(fh, cll[kc], ac[b]);
+                ac[b] = mfma16<false>(fl, chh[kc], ac[b]);
+            }
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+        for (int kc = 0; kc < MB2; ++kc) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+            if (kc + 1 < MB2) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int nb = 2 * g + b;
+            if (nb < NC) {  // uniform
+                float e[4], dm[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    e[r] = fmaf(-ac[b][r], ars[16 * nb + 4 * h + r] * ics, sreg[nb][r]);
+                    dm[r] = sreg[nb][r] - mean;
+                }
+                if (nb + 1 == NC)  // uniform
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        e[r] *= lastm[r];
+                        dm[r] *= lastm[r];
+                    }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    rss = fmaf(e[r], e[r], rss);
+                    sst = fmaf(dm[r], dm[r], sst);
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        st.template store<1 - P>(fb[1 - P], tid);
+        __syncthreads();
+    };
+#pragma unroll
+    for (int g = 0; g < GM; g += 2) {
+        chunk(g, std::integral_constant<int, 0>{});
+        if (g + 1 < GM) chunk(g + 1, std::integral_constant<int, 1>{});
+    }
+    rss = lane_sum4(rss);
+    sst = lane_sum4(sst);
+    if (!valid) return;
+    const uint8_t st_v = status[v];
+    write_maps(a, out_idx, h, nnz, rss, sst, its[v], st_v & 1, double(alpha_v[v]), (st_v & 2) != 0);
+}
+
+template <int MB>
+void launch_epilogue(const FistaArgs& args, const float* X, const uint32_t* its, const uint8_t* status,
+                     const float* alpha_v, const uint16_t* AH16, const float* Ainv16, hipStream_t s) {
+    const int64_t tiles = (args.n_vox + 15) / 16;
+    if constexpr (MB <= kTik16MaxMB) {
+        if (AH16 && args.NC <= kTik16MaxBlocks) {
+            constexpr int W = 4;
+            hipLaunchKernelGGL((epilogue_h16_kernel<MB, kTik16MaxBlocks, W>), dim3(unsigned((tiles + W - 1) / W)),
+                               dim3(64 * W), 0, s, args, reinterpret_cast<const floatx4*>(AH16), Ainv16, X, its,
+                               status, alpha_v);
+            SF_HIP(hipGetLastError());
+            return;
+        }
+    }
+    constexpr int W = 4;
+    hipLaunchKernelGGL((epilogue_lds_kernel<MB, W>), dim3(unsigned((tiles + W - 1) / W)), dim3(64 * W), 0, s, args, X,
+                       its, status, alpha_v);
+    SF_HIP(hipGetLastError());
+}
+
 template <int MB>
 void launch_tik(const FistaArgs& args, const TikhonovLaunch& L, hipStream_t s) {
     const int64_t n_vox = L.n_vox;
@@ -2192,7 +2371,7 @@ void launch_f32_persist(const FistaArgs& args, const FistaLaunch& L, hipStream_t
         return;
     }
     if (!L.Xbuf) return;
-    launch_epilogue<MB>(args, L.Xbuf, L.itsbuf, L.statusbuf, L.alphabuf, s);
+    launch_epilogue<MB>(args, L.Xbuf, L.itsbuf, L.statusbuf, L.alphabuf, L.AH16, L.Ainv16, s);
 }
 
 template <int MB, bool BF16>
@@ -2265,7 +2444,7 @@ void launch_mixed_mb(const FistaLaunch& L, const FistaArgs& ea, hipStream_t s) {
         launch_path_select<MB>(ea, s);
         return;
     }
-    launch_epilogue<MB>(ea, L.Xbuf, L.itsbuf, L.statusbuf, L.alphabuf, s);
+    launch_epilogue<MB>(ea, L.Xbuf, L.itsbuf, L.statusbuf, L.alphabuf, L.AH16, L.Ainv16, s);
 }
 
 }  // namespace

@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(64) tik16_scales_kernel(const double* __restri
     const double* row = nullptr;
     int len = 0;
     if (t < Kp) {
-        if (t < K) row = P + size_t(t) * N, len = N;
+        if (t < K && P) row = P + size_t(t) * N, len = N;  // P == nullptr: A's chunks only
     } else if (t - Kp < N) {
         row = A + size_t(t - Kp) * K, len = K;
     }
