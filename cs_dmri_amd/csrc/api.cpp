@@ -228,8 +228,48 @@ struct Design {
     int N = 0, K = 0, Np = 0, Kp = 0;
     DevBuf<double> A;   // N x K row-major, FP64
     DevBuf<double> G;   // K x K
+    DevBuf<double> qv;  // q and the gradient directions the design was built from (4N)
     double L = 0.0;     // Lipschitz constant sigma_max^2 / N (floored at eps)
 };
+
+// Per-thread pinned host staging for the small uploads of a call (q and the
+// gradient directions, the momentum table), so that they are true
+// asynchronous copies: a copy from pageable memory would wait for the work
+// already queued on the stream, which leaves the device idle while the host
+// catches up. Before a buffer is rewritten the host waits for the event
+// recorded after its previous copy (normally long complete).
+template <int TAG>
+struct PinnedStage {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipEvent_t done = nullptr;
+    ~PinnedStage() {
+        if (done) (void)hipEventSynchronize(done), (void)hipEventDestroy(done);
+        if (p) (void)hipHostFree(p);
+    }
+    // host memory of at least n bytes, free for writing
+    void* get(size_t n) {
+        if (done) SF_HIP(hipEventSynchronize(done));
+        if (n > bytes) {
+            if (p) SF_HIP(hipHostFree(p));
+            p = nullptr;
+            bytes = 0;
+            SF_HIP(hipHostMalloc(&p, n));
+            bytes = n;
+        }
+        return p;
+    }
+    // after the copy out of the buffer is queued on s
+    void queued(hipStream_t s) {
+        if (!done) SF_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        SF_HIP(hipEventRecord(done, s));
+    }
+};
+template <int TAG>
+PinnedStage<TAG>& pinned_stage() {
+    thread_local PinnedStage<TAG> st;
+    return st;
+}
 
 void upload_design(Design& d, const double* A_host, int N, int K, hipStream_t s, bool mixed = false) {
     d.N = N;
@@ -256,16 +296,17 @@ void build_design(Design& d, const ShorefitGradients& g, double big, double smal
                     "radial order " + std::to_string(order) + " (" + std::to_string(K) +
                         " coefficients) exceeds the supported maximum of 10 (256 coefficients)"};
     // src/qspace.rs:130-149 — q = sqrt(b / (4 pi^2 tau)), 0 at or below the b0 threshold.
-    std::vector<double> q(N);
+    auto& stage = pinned_stage<0>();
+    double* q = static_cast<double*>(stage.get(sizeof(double) * 4 * N));  // q, then the directions
     const double tau = big - small / 3.0;
     const double denom = 4.0 * kPi * kPi * tau;
     for (int i = 0; i < N; ++i) q[i] = (g.bvals[i] <= g.b0_threshold) ? 0.0 : std::sqrt(g.bvals[i] / denom);
-    DevBuf<double> dq(N), dv(size_t(3) * N);
-    SF_HIP(hipMemcpyAsync(dq.p, q.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    SF_HIP(hipMemcpyAsync(dv.p, g.bvecs, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+    std::memcpy(q + N, g.bvecs, sizeof(double) * 3 * N);
+    d.qv.alloc(size_t(4) * N);  // kept with the design: no wait for the kernel here
+    SF_HIP(hipMemcpyAsync(d.qv.p, q, sizeof(double) * 4 * N, hipMemcpyHostToDevice, s));
+    stage.queued(s);
     d.A.alloc(size_t(N) * K);
-    launch_design_matrix(dq.p, dv.p, N, order, zeta, d.A.p, s);
-    SF_HIP(hipStreamSynchronize(s));  // dq / dv are released at scope exit
+    launch_design_matrix(d.qv.p, d.qv.p + N, N, order, zeta, d.A.p, s);
 }
 
 struct FistaOps {
@@ -364,16 +405,18 @@ int refresh_period(int requested, int precision, int Kp) {
 // Momentum table (t_k - 1) / t_{k+1}, t_0 = 1 (src/solver/fista.rs:111-112),
 // in FP64, rounded to FP32; max(max_iter, 1) entries, on the device.
 void upload_momentum(DevBuf<float>& mom, uint32_t max_iter, hipStream_t s) {
-    std::vector<float> host(std::max<uint32_t>(max_iter, 1));
+    const size_t n = std::max<uint32_t>(max_iter, 1);
+    auto& stage = pinned_stage<1>();
+    float* host = static_cast<float*>(stage.get(sizeof(float) * n));
     double t = 1.0;
-    for (size_t k = 0; k < host.size(); ++k) {
+    for (size_t k = 0; k < n; ++k) {
         const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
         host[k] = float((t - 1.0) / tn);
         t = tn;
     }
-    mom.alloc(host.size());
-    SF_HIP(hipMemcpyAsync(mom.p, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice, s));
-    SF_HIP(hipStreamSynchronize(s));  // the host table goes out of scope
+    mom.alloc(n);
+    SF_HIP(hipMemcpyAsync(mom.p, host, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    stage.queued(s);
 }
 
 // FISTA set-up (src/solver/fista.rs:38-62) and right-hand sides. The caller
@@ -855,43 +898,10 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
                             "): without L1 regularization the fit is underdetermined and its fit statistics "
                             "are not meaningful; use a lower radial order or L1");
 
-    // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
-    // A device mask is compacted on the device; a host mask on the host;
-    // without a mask every voxel is fitted in place (no index list).
-    const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
-    DevBuf<int64_t> d_vox;
-    int64_t V = 0;
-    if (!vol->mask) {
-        V = nvox_total;
-    } else if (vol->on_device) {
-        V = compact_mask(vol->mask, nvox_total, d_vox, s);
-    } else {
-        std::vector<int64_t> vox;
-        vox.reserve(nvox_total);
-        for (int64_t i = 0; i < nvox_total; ++i)
-            if (vol->mask[i]) vox.push_back(i);
-        V = int64_t(vox.size());
-        d_vox.alloc(V);
-        if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
-        SF_HIP(hipStreamSynchronize(s));  // vox is released at scope exit
-    }
-    const int64_t* vox_dev = d_vox.p;  // null: voxel v is row v
-    out->n_voxels = V;
-
-    // --- gradient deviation: identity voxels keep the shared design -------
-    const int64_t* vox_shared = vox_dev;
-    int64_t V_shared = V;
-    DevBuf<float> d_field;
-    DevBuf<int64_t> d_vox_id, d_vox_dv;
-    const float* field = nullptr;
-    int64_t V_dv = 0;
-    if (gd) {
-        field = dev_in(gd->field, size_t(nvox_total) * 9, vol->on_device != 0, d_field, s);
-        V_shared = gd_partition(field, vox_dev, V, d_vox_id, d_vox_dv, &V_dv, s);
-        vox_shared = d_vox_id.p;
-    }
-
     // --- device buffers -------------------------------------------------
+    // Queued ahead of the mask compaction, whose voxel count the host waits
+    // for: the device clears the outputs and builds the design meanwhile.
+    const int64_t nvox_total = vol->nx * vol->ny * vol->nz;
     const size_t coef_bytes = sizeof(float) * size_t(nvox_total) * K;
     DevBuf<float> d_data, d_coef, d_r2, d_res, d_alpha, d_bic, d_rssl2;
     DevBuf<uint32_t> d_it;
@@ -950,6 +960,41 @@ void fit_impl(const ShorefitGradients* grads, const ShorefitVolume* vol, const S
     const int precision = (l1 && sp.max_iter > 0) ? sp.precision : SHOREFIT_PREC_FP32;
     Design d;
     build_design(d, *grads, big, small, sp.radial_order, sp.zeta, s, precision != SHOREFIT_PREC_FP32);
+    // --- masked voxels in x -> y -> z order (src/voxel_loop.rs:28-42) --
+    // A device mask is compacted on the device; a host mask on the host;
+    // without a mask every voxel is fitted in place (no index list).
+    DevBuf<int64_t> d_vox;
+    int64_t V = 0;
+    if (!vol->mask) {
+        V = nvox_total;
+    } else if (vol->on_device) {
+        V = compact_mask(vol->mask, nvox_total, d_vox, s);
+    } else {
+        std::vector<int64_t> vox;
+        vox.reserve(nvox_total);
+        for (int64_t i = 0; i < nvox_total; ++i)
+            if (vol->mask[i]) vox.push_back(i);
+        V = int64_t(vox.size());
+        d_vox.alloc(V);
+        if (V) SF_HIP(hipMemcpyAsync(d_vox.p, vox.data(), sizeof(int64_t) * V, hipMemcpyHostToDevice, s));
+        SF_HIP(hipStreamSynchronize(s));  // vox is released at scope exit
+    }
+    const int64_t* vox_dev = d_vox.p;  // null: voxel v is row v
+    out->n_voxels = V;
+
+    // --- gradient deviation: identity voxels keep the shared design -------
+    const int64_t* vox_shared = vox_dev;
+    int64_t V_shared = V;
+    DevBuf<float> d_field;
+    DevBuf<int64_t> d_vox_id, d_vox_dv;
+    const float* field = nullptr;
+    int64_t V_dv = 0;
+    if (gd) {
+        field = dev_in(gd->field, size_t(nvox_total) * 9, vol->on_device != 0, d_field, s);
+        V_shared = gd_partition(field, vox_dev, V, d_vox_id, d_vox_dv, &V_dv, s);
+        vox_shared = d_vox_id.p;
+    }
+
     hipEvent_t e0, e1, e2, e3;
     SF_HIP(hipEventCreate(&e0));
     SF_HIP(hipEventCreate(&e1));
