@@ -7,18 +7,20 @@ chunks with zeros past the signal, and keeps the signals in registers; the
 last sample block of the signal is partial unless 16 divides N. Longer
 signals, and padded K above 160 (order 10), go to the FP32 kernels
 (`tikhonov_lds_kernel` up to 288 measurements, then the kernels that read the
-signals twice). `epilogue_lds_kernel` (residual, R^2, BIC of the persistent
-solvers) stages A by chunks of 16 rows with no bound on N. The cases cover
-one block (N <= 16), a block plus one sample, odd and even block counts, the
-largest register case, the fallback past it, and order 10 on both sides of
-288 measurements.
+signals twice). The epilogue of the persistent solvers (residual, R^2, BIC)
+is the same split-FP16 residual pass (`epilogue_h16_kernel`, N <= 288) or,
+past 288 measurements, the FP32 `epilogue_lds_kernel`, which stages A by
+chunks of 16 rows with no bound on N. The cases cover one block (N <= 16), a
+block plus one sample, odd and even block counts, the largest register case,
+the fallbacks past it, and order 10 on both sides of 288 measurements.
 
 Tolerances: Tikhonov coefficients and residuals against the FP64 oracle
 (TikhonovSolver::fit, src/solver/tikhonov.rs:59-76) at 1e-4 relative, as the
 order-8 tests; the epilogue's residual, R^2 and BIC against the same maps
 recomputed in FP64 (src/fit.rs:120-128, src/solver/alpha.rs:99-103) from the
 device's own coefficients, so the check isolates the epilogue from the
-solver: residual 1e-5 relative (FP32 products accumulated over N samples).
+solver: residual 1e-5 relative (products of ~22 bits, FP32 sums over N
+samples).
 """
 
 import numpy as np
