@@ -241,16 +241,17 @@ __global__ void __launch_bounds__(256) cholesky_kernel(double* __restrict__ G, i
 }
 
 // The same right-looking factorization (the same operations in the same
-// order, so the same bits) on the packed lower triangle held in LDS
+// order, the column scaling by the pivot's reciprocal aside) on the packed
+// lower triangle held in LDS
 // (L[a][b] at a (a + 1) / 2 + b): the trailing updates are LDS traffic
 // instead of L2 round trips. Used when K (K + 1) / 2 doubles fit in LDS.
 // Blocked right-looking Cholesky on the packed lower triangle in LDS: panels
 // of kPanel columns. Wave 0 factors a panel on its own, in registers (no
-// workgroup barrier),
-// then every wave subtracts the panel's columns from the trailing triangle,
-// one column after another, so each entry receives exactly the updates of
-// the column-by-column algorithm in the same order (the same L bit for bit),
-// with two workgroup barriers per panel instead of three per column.
+// workgroup barrier; a column's entries are scaled by the pivot's
+// reciprocal), then every wave subtracts the panel's columns from the
+// trailing triangle, one column after another, so each entry receives the
+// updates of the column-by-column algorithm in the same order, with two
+// workgroup barriers per panel instead of three per column.
 constexpr int kPanel = 8;
 constexpr int kPanelRows = 3;  // rows per lane in the panel: K <= 192
 
@@ -299,14 +300,18 @@ __global__ void __launch_bounds__(1024) cholesky_lds_kernel(double* __restrict__
                     fail = true;
                     continue;
                 }
+                // Branch-free: one reciprocal of the pivot per column and
+                // selects instead of per-row branches (exec-masked FP64
+                // divisions had made each column ~1 us of latency).
                 const double sj = sqrt(d);
+                const double rj = 1.0 / sj;
 #pragma unroll
                 for (int q = 0; q < kPanelRows; ++q) {
                     const int i = j0 + lane + 64 * q;
-                    if (i > j && i < K) pv[q][c] /= sj;
-                    else if (i == j) pv[q][c] = sj;
+                    const double sc = pv[q][c] * rj;
+                    pv[q][c] = i > j ? sc : (i == j ? sj : pv[q][c]);
                 }
-                // the panel's later columns, rows k..K-1
+                // the panel's later columns, rows k..K-1 (rows past K hold zeros)
 #pragma unroll
                 for (int k = c + 1; k < kPanel; ++k) {
                     if (k >= jb) continue;
@@ -314,7 +319,8 @@ __global__ void __launch_bounds__(1024) cholesky_lds_kernel(double* __restrict__
 #pragma unroll
                     for (int q = 0; q < kPanelRows; ++q) {
                         const int i = j0 + lane + 64 * q;
-                        if (i >= j0 + k && i < K) pv[q][k] -= pv[q][c] * lkj;
+                        const double u = fma(-pv[q][c], lkj, pv[q][k]);
+                        pv[q][k] = i >= j0 + k ? u : pv[q][k];
                     }
                 }
             }
