@@ -312,58 +312,113 @@ __device__ __forceinline__ float key_float(uint32_t k) {
 }
 
 // Fit-failure thresholds (src/io/microstructure.rs:215-268, 303-326): for
-// map q (one workgroup each), factor x the value of rank round((n - 1) 0.99)
-// among the n finite values in ascending order, found by a four-pass radix
-// selection over 8-bit digits of the order-preserving keys (the same element
-// a full sort, or nth_element, puts at that rank). No finite value: inactive.
-__global__ void __launch_bounds__(1024) micro_threshold_kernel(const float* __restrict__ raw, int64_t V, float factor,
-                                                               MicroThresholds* __restrict__ th) {
-    const int q = blockIdx.x;
+// each map q, factor x the value of rank round((n - 1) 0.99) among the n
+// finite values in ascending order, found by a four-pass radix selection over
+// 8-bit digits of the order-preserving keys (the same element a full sort, or
+// nth_element, puts at that rank). No finite value: inactive.
+// The selection runs over kSelSlices workgroups per map (one workgroup per
+// map, all passes in one launch, kept six CUs busy: 0.39 ms for the six maps
+// of the whole-brain volume). SelState holds each map's count, rank, digit prefix and the
+// histogram of the current pass in device memory; `sel_hist_kernel` adds a
+// slice's histogram into it, `sel_pick_kernel` takes the digit and clears it.
+struct SelState {
+    unsigned long long n;
+    long long rank;
+    uint32_t prefix, mask;
+    unsigned hist[256];
+};
+constexpr int kSelSlices = 32;  // workgroups per map
+
+__global__ void sel_init_kernel(SelState* __restrict__ st) {
+    SelState& S = st[blockIdx.x];
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) S.hist[b] = 0u;
+    if (threadIdx.x == 0) {
+        S.n = 0ull;
+        S.rank = 0;
+        S.prefix = 0u;
+        S.mask = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) sel_count_kernel(const float* __restrict__ raw, int64_t V,
+                                                        SelState* __restrict__ st) {
+    const int q = blockIdx.y;
     const float* x = raw + q * V;
     __shared__ unsigned long long cnt;
-    __shared__ unsigned hist[256];
-    __shared__ uint32_t prefix_s, mask_s;
-    __shared__ long long rank_s;
-    if (threadIdx.x == 0) cnt = 0;
+    if (threadIdx.x == 0) cnt = 0ull;
     __syncthreads();
     unsigned long long c = 0;
-    for (int64_t v = threadIdx.x; v < V; v += blockDim.x) c += isfinite(x[v]) ? 1 : 0;
+    for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += int64_t(gridDim.x) * blockDim.x)
+        c += isfinite(x[v]) ? 1 : 0;
     atomicAdd(&cnt, c);
     __syncthreads();
-    const long long n = (long long)cnt;
+    if (threadIdx.x == 0 && cnt) atomicAdd(&st[q].n, cnt);
+}
+
+__global__ void __launch_bounds__(256) sel_hist_kernel(const float* __restrict__ raw, int64_t V, int shift,
+                                                       SelState* __restrict__ st) {
+    const int q = blockIdx.y;
+    const float* x = raw + q * V;
+    __shared__ unsigned hist[256];
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0u;
+    __syncthreads();
+    const uint32_t prefix = st[q].prefix, mask = st[q].mask;
+    for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += int64_t(gridDim.x) * blockDim.x) {
+        const float f = x[v];
+        if (!isfinite(f)) continue;
+        const uint32_t k = float_key(f);
+        if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 256; b += blockDim.x)
+        if (hist[b]) atomicAdd(&st[q].hist[b], hist[b]);
+}
+
+// One wave per map: lane l holds bins 4l..4l+3; the digit is the first bin
+// whose running count passes the rank (capped at 255), found with a prefix
+// sum across the lanes instead of a serial scan of 256 bins.
+__global__ void __launch_bounds__(64) sel_pick_kernel(int shift, float factor, SelState* __restrict__ st,
+                                                      MicroThresholds* __restrict__ th) {
+    const int q = blockIdx.x, lane = threadIdx.x;
+    SelState& S = st[q];
+    const long long n = (long long)S.n;
     if (n == 0) {
-        if (threadIdx.x == 0) th->active[q] = 0;
+        if (lane == 0) th->active[q] = 0;
         return;
     }
-    if (threadIdx.x == 0) {
-        rank_s = std::min((long long)llround(0.99 * double(n - 1)), n - 1);
-        prefix_s = 0u;
-        mask_s = 0u;
+    const long long r0 = shift == 24 ? std::min((long long)llround(0.99 * double(n - 1)), n - 1) : S.rank;
+    unsigned hb[4];
+    long long sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        hb[i] = S.hist[4 * lane + i];
+        sum += hb[i];
     }
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0u;
-        __syncthreads();
-        const uint32_t prefix = prefix_s, mask = mask_s;
-        for (int64_t v = threadIdx.x; v < V; v += blockDim.x) {
-            const float f = x[v];
-            if (!isfinite(f)) continue;
-            const uint32_t k = float_key(f);
-            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long r = rank_s;
-            int d = 0;
-            while (d < 255 && r >= (long long)hist[d]) r -= hist[d++];
-            rank_s = r;
-            prefix_s = prefix | (uint32_t(d) << shift);
-            mask_s = mask | (255u << shift);
-        }
-        __syncthreads();
+    long long incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
     }
-    if (threadIdx.x == 0) {
+    const unsigned long long past = __ballot(incl > r0);
+    const int sel = past ? __builtin_ctzll(past) : 63;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) S.hist[4 * lane + i] = 0u;  // cleared for the next pass
+    if (lane != sel) return;
+    long long r = r0 - (incl - sum);
+    int d = 4 * lane;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (d == 255 || r < (long long)hb[i]) break;
+        r -= hb[i];
+        ++d;
+    }
+    S.rank = r;
+    S.prefix |= uint32_t(d) << shift;
+    S.mask |= 255u << shift;
+    if (shift == 0) {
         th->active[q] = 1;
-        th->thr[q] = factor * key_float(prefix_s);
+        th->thr[q] = factor * key_float(S.prefix);
     }
 }
 
@@ -474,7 +529,18 @@ void launch_micro(const float* coef, const int64_t* vox, int64_t V, int K, const
 
 void launch_micro_thresholds(const float* raw, int64_t V, float factor, MicroThresholds* d_th, hipStream_t s) {
     if (V <= 0) return;
-    hipLaunchKernelGGL(micro_threshold_kernel, dim3(6), dim3(1024), 0, s, raw, V, factor, d_th);
+    DevBuf<SelState> st(6);
+    hipLaunchKernelGGL(sel_init_kernel, dim3(6), dim3(256), 0, s, st.p);
+    SF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sel_count_kernel, dim3(kSelSlices, 6), dim3(256), 0, s, raw, V, st.p);
+    SF_HIP(hipGetLastError());
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(sel_hist_kernel, dim3(kSelSlices, 6), dim3(256), 0, s, raw, V, shift, st.p);
+        SF_HIP(hipGetLastError());
+        hipLaunchKernelGGL(sel_pick_kernel, dim3(6), dim3(64), 0, s, shift, factor, st.p, d_th);
+        SF_HIP(hipGetLastError());
+    }
+    sync_if_unscoped(s);  // st is released at scope exit
     SF_HIP(hipGetLastError());
 }
 
