@@ -1410,7 +1410,7 @@ __global__ void __launch_bounds__(64 * W, 2) tikhonov_h16_kernel(const Tik16Args
             smax = fmaxf(smax, fabsf(sreg[ch][r]));
             sum += sreg[ch][r];
         }
-    const float ss = pow2_to(lane_max4(smax), 15);
+    const float ss = fminf(pow2_to(lane_max4(smax), 15), 0x1p100f);  // finite for any signal
     __syncthreads();
     // one step of the chunk pipeline: request chunk g + 2 into the register
     // set chunk g came from, multiply chunk g, store chunk g + 1
@@ -1483,7 +1483,7 @@ __global__ void __launch_bounds__(64 * W, 2) tikhonov_h16_kernel(const Tik16Args
                 const int k = 16 * mb + 4 * h + r;
                 if (k < a.K) a.coef[out_idx * a.K + k] = c[mb][r];
             }
-    const float cs = pow2_to(lane_max4(cm), 15), ics = 1.f / cs;
+    const float cs = fminf(pow2_to(lane_max4(cm), 15), 0x1p100f), ics = 1.f / cs;
     uint4 chh[MB2], cll[MB2];
 #pragma unroll
     for (int kc = 0; kc < MB2; ++kc) {
@@ -1629,7 +1629,7 @@ __global__ void __launch_bounds__(64 * W, 2) epilogue_h16_kernel(const FistaArgs
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) xm = fmaxf(xm, fabsf(x[mb][r]));
-    const float cs = pow2_to(lane_max4(xm), 15), ics = 1.f / cs;
+    const float cs = fminf(pow2_to(lane_max4(xm), 15), 0x1p100f), ics = 1.f / cs;
     uint4 chh[MB2], cll[MB2];
 #pragma unroll
     for (int kc = 0; kc < MB2; ++kc) {

@@ -452,7 +452,9 @@ __global__ void __launch_bounds__(64) tik16_scales_kernel(const double* __restri
     if (lane != 0) return;
     int e = 0;
     (void)frexp(m, &e);
-    inv[t] = m > 0.0 ? float(ldexp(1.0, e - 15)) : 1.f;
+    // the inverse scale stays a normal FP32 number (rows below ~2^-111 keep
+    // the smallest scale instead of flushing to zero)
+    inv[t] = m > 0.0 ? float(ldexp(1.0, std::min(std::max(e - 15, -126), 127))) : 1.f;
 }
 
 // The fragments, chunk by chunk: chunks g < G1e hold, for coefficient block

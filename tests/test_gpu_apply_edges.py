@@ -91,3 +91,21 @@ def test_epilogue_maps_chunk_counts(orc, precision, N, order):
     bic = N * np.log(np.maximum(rss, np.finfo(float).tiny) / N) + np.log(N) * nnz
     np.testing.assert_allclose(out["bic"][mask], bic, rtol=1e-5, atol=1e-2)
     assert not out["residual_l2"][~mask].any() and not out["coefficients"][~mask].any()
+
+
+@pytest.mark.parametrize("scale", [1e-15, 1e15])
+def test_tikhonov_batch_signal_scales(orc, scale):
+    # the split-FP16 apply scales each voxel's signals and coefficients by a
+    # power of two of their own magnitude: far from 1 the results follow the
+    # signals' scale exactly as the FP64 oracle's do
+    rng = np.random.default_rng(7)
+    N, order = 258, 8
+    K = orc.n_coeffs(order)
+    A = rng.standard_normal((N, K))
+    dn, dl = orc.regularization(order)
+    S = ((A @ rng.standard_normal((K, 40))).T + 0.1 * rng.standard_normal((40, N))) * scale
+    C, R = sf.tikhonov_batch(A, S, dn, dl, 1e-2, 1e-2)
+    Cr, Rr = orc.tikhonov(A, S.astype(np.float32).astype(np.float64), dn, dl, 1e-2, 1e-2)
+    assert np.isfinite(C).all() and np.isfinite(R).all()
+    assert _rel(C.astype(np.float64), Cr).max() <= 1e-4
+    np.testing.assert_allclose(R, Rr, rtol=1e-4)
