@@ -50,7 +50,7 @@ def test_laguerre_kats(orc):
 
 
 def test_coefficient_counts(orc):
-    # src/basis/shore.rs:211-219 and BASELINE's orders 6/8/10.
+    # src/basis/shore.rs:211-219; the radial orders 6, 8 and 10 of the fits.
     assert orc.n_coeffs(6) == 72 and orc.n_coeffs(2) == 8
     assert orc.n_coeffs(8) == 145 and orc.n_coeffs(10) == 256
     idx = orc.shore_indices(4)
